@@ -1,0 +1,644 @@
+"""CPU oracle for VULCAN's Ros2 kinetics-plus-diffusion step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the ``vulcan_amd`` package imports this
+module; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` use it, as the checker and as the timed CPU baseline.
+
+It restates the reference algorithm in numpy, keeping the reference's
+expression trees (operation order) wherever that is cheap, so that the device
+kernels can be compared nearly bit-for-bit:
+
+* rate constants                      op.py:145-207, 287-309; chem_funs.py:2769-2796
+* ``chemdf``                          make_chem_funs.py:199-316 (generated chem_funs.py:931-1000)
+* analytic Jacobian (``symjac``)      make_chem_funs.py:653-717
+* ``diffdf`` / ``diffdf_no_mol``      op.py:1496-1597 / 1438-1494
+* ``lhs_jac_tot`` / ``lhs_jac_no_mol`` op.py:1973-2042 / 2122-2166
+* ``store_bandM`` + two ``solve_banded`` calls   op.py:2833-2858, 2914, 2929
+* ``Ros2.solver``                     op.py:2860-3007
+* ``clip``/``loss``/``step_ok``/``step_reject``/``reset_y``  op.py:2447-2534
+* ``one_step`` / ``step_size``        op.py:3091-3125
+* ``Integration`` loop, ``f_dy``, ``conv``, ``stop``, ``save_step``,
+  ``update_mu_dz``                    op.py:808-1105, 944-984
+
+Parity pins for each part are listed in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+import scipy.linalg
+
+from vulcan_amd.config import Config, kb, Navo
+from vulcan_amd import network as nw
+
+R_ROS = 1. + 1. / 2. ** 0.5       # op.py:2890
+CORR = kb / 1.e6                   # gibbs_text.txt:1 (P0 = 1 bar)
+
+
+# ---------------------------------------------------------------------------
+# A1: rate constants
+# ---------------------------------------------------------------------------
+
+def h_RT(T, a):
+    """thermo/gibbs_text.txt:13-14 (chem_funs.py:2781-2782)."""
+    return -a[0] / T ** 2 + a[1] * np.log(T) / T + a[2] + a[3] * T / 2. + a[4] * T ** 2 / 3. \
+        + a[5] * T ** 3 / 4. + a[6] * T ** 4 / 5. + a[8] / T
+
+
+def s_R(T, a):
+    """thermo/gibbs_text.txt:17-18."""
+    return -a[0] / T ** 2 / 2. - a[1] / T + a[2] * np.log(T) + a[3] * T + a[4] * T ** 2 / 2. \
+        + a[5] * T ** 3 / 3. + a[6] * T ** 4 / 4. + a[9]
+
+
+def g_RT(T, nasa9_row):
+    """Low set below 1000 K, high set at and above (chem_funs.py:2789-2792)."""
+    lo, hi = nasa9_row[:10], nasa9_row[10:20]
+    return (T < 1000) * (h_RT(T, lo) - s_R(T, lo)) + (T >= 1000) * (h_RT(T, hi) - s_R(T, hi))
+
+
+def gibbs_K(net: nw.Network, p: int, T):
+    """K_eq of pair p (make_chem_funs.py:568-580 as emitted at chem_funs.py:2810ff).
+
+    exp(-(sum_prod g - sum_reac g)) * (corr*T)**(n_reac - n_prod), summed
+    left to right as the generated string does: reactants with '-1*', then
+    products with '+1*'; M is excluded from the counts.
+    """
+    acc = None
+    nre = npr = 0
+    for t in net.reac[p]:
+        if t >= 0:
+            term = -1 * g_RT(T, net.nasa9[t])
+            acc = term if acc is None else acc + term
+            nre += 1
+    for t in net.prod[p]:
+        if t >= 0:
+            term = 1 * g_RT(T, net.nasa9[t])
+            acc = term if acc is None else acc + term
+            npr += 1
+    K = np.exp(-(acc))
+    if npr - nre != 0:
+        K = K * (CORR * T) ** (nre - npr)
+    return K
+
+
+def forward_rate(net: nw.Network, p: int, T, M, tau=None):
+    kind = int(net.kind[p])
+    par = net.par[p]
+    A, n, E = par[nw.P_A], par[nw.P_N], par[nw.P_E]
+    if kind in (nw.KIND_ARRH, nw.KIND_K0):
+        return A * T ** n * np.exp(-E / T)                                   # op.py:166
+    if kind in (nw.KIND_LIND, nw.KIND_TROE):
+        k0 = A * T ** n * np.exp(-E / T)
+        kinf = par[nw.P_AINF] * T ** par[nw.P_NINF] * np.exp(-par[nw.P_EINF] / T)   # op.py:179
+        k = k0 / (1 + k0 * M / kinf)                                         # op.py:181
+        if kind == nw.KIND_LIND:
+            return k
+        Fc = par[nw.P_FC]
+        nn = 0.75 - 1.27 * np.log(Fc)                                        # op.py:204
+        ff = np.exp(np.log(Fc) / (1. + (np.log(k0 * M / kinf) / nn) ** 2))   # op.py:205
+        return k0 / (1 + k0 * M / kinf) * ff                                 # op.py:207
+    if kind == nw.KIND_PHOTO:
+        return par[nw.P_J0] * np.exp(-tau)
+    if kind == nw.KIND_CH3OH:                                                # op.py:197-207
+        k0 = 1.932E3 * T ** -9.88 * np.exp(-7544. / T) + 5.109E-11 * T ** -6.25 * np.exp(-1433. / T)
+        kinf = 1.031E-10 * T ** -0.018 * np.exp(16.74 / T)
+        Fc = 0.1855 * np.exp(-T / 155.8) + 0.8145 * np.exp(-T / 1675.) + np.exp(-4531. / T)
+        nn = 0.75 - 1.27 * np.log(Fc)
+        ff = np.exp(np.log(Fc) / (1. + (np.log(k0 * M / kinf) / nn) ** 2))
+        return k0 / (1 + k0 * M / kinf) * ff
+    raise ValueError(kind)
+
+
+def rates(net: nw.Network, T, M, tau=None) -> np.ndarray:
+    """k[l, 2p] forward, k[l, 2p+1] reverse = k_f / K_eq or 0 (op.py:287-309)."""
+    nz = T.shape[0]
+    k = np.zeros((nz, net.nr))
+    for p in range(net.npairs):
+        kf = forward_rate(net, p, T, M, tau)
+        k[:, 2 * p] = kf
+        if net.rev[p]:
+            k[:, 2 * p + 1] = kf / gibbs_K(net, p, T)
+    return k
+
+
+# ---------------------------------------------------------------------------
+# A2/A3: chemistry right-hand side and Jacobian
+# ---------------------------------------------------------------------------
+
+def _side_product(terms, kcol, y, M):
+    acc = kcol
+    for t in terms:
+        if t == nw.TERM_NONE:
+            continue
+        acc = acc * (M if t == nw.TERM_M else y[:, t])
+    return acc
+
+
+def pair_rates(net, y, M, k):
+    """v_p per layer, evaluated left to right (make_chem_funs.py:293-315)."""
+    v = np.empty((y.shape[0], net.npairs))
+    for p in range(net.npairs):
+        v[:, p] = _side_product(net.reac[p], k[:, 2 * p], y, M) \
+            - _side_product(net.prod[p], k[:, 2 * p + 1], y, M)
+    return v
+
+
+def chemdf(net, y, M, k, tables=None) -> np.ndarray:
+    """dydt (nz, ni): -v per reactant occurrence, +v per product occurrence,
+    summed in reaction order (chem_funs.py:931-1000)."""
+    tables = tables or nw.build_tables(net)
+    v = pair_rates(net, y, M, k)
+    out = np.zeros_like(y)
+    for i in range(net.ni):
+        lo, hi = tables.chem_ptr[i], tables.chem_ptr[i + 1]
+        if lo == hi:
+            continue
+        acc = tables.chem_sign[lo] * v[:, tables.chem_pair[lo]]
+        for e in range(lo + 1, hi):
+            acc = acc + tables.chem_sign[e] * v[:, tables.chem_pair[e]]
+        out[:, i] = acc
+    return out
+
+
+def _dside(terms, kcol, y, M, m):
+    """d/dy_m of kcol * prod(terms)."""
+    tot = None
+    for q, t in enumerate(terms):
+        if t != m:
+            continue
+        acc = kcol
+        for q2, t2 in enumerate(terms):
+            if q2 == q or t2 == nw.TERM_NONE:
+                continue
+            acc = acc * (M if t2 == nw.TERM_M else y[:, t2])
+        tot = acc if tot is None else tot + acc
+    return 0.0 if tot is None else tot
+
+
+def jac_blocks(net, y, M, k, tables=None) -> np.ndarray:
+    """J[l, i, m] = d chemdf[l, i] / d y[l, m] (make_chem_funs.py:653-684)."""
+    tables = tables or nw.build_tables(net)
+    nz, ni = y.shape
+    dv = np.empty((nz, tables.nslot))
+    for s in range(tables.nslot):
+        p, m = tables.slot_pair[s], tables.slot_sp[s]
+        dv[:, s] = _dside(net.reac[p], k[:, 2 * p], y, M, m) - _dside(net.prod[p], k[:, 2 * p + 1], y, M, m)
+    J = np.zeros((nz, ni, ni))
+    for e_dense in np.nonzero(tables.jac_map >= 0)[0]:
+        e = tables.jac_map[e_dense]
+        i, m = divmod(int(e_dense), ni)
+        lo, hi = tables.jac_ptr[e], tables.jac_ptr[e + 1]
+        acc = tables.jac_coef[lo] * dv[:, tables.jac_slot[lo]]
+        for q in range(lo + 1, hi):
+            acc = acc + tables.jac_coef[q] * dv[:, tables.jac_slot[q]]
+        J[:, i, m] = acc
+    return J
+
+
+def neg_symjac_dense(J: np.ndarray) -> np.ndarray:
+    """Dense (ni*nz)^2 -J with index l*ni+i, one fancy-index write per (i, m)
+    as the generated neg_symjac does (chem_funs.py:7456-7461)."""
+    nz, ni, _ = J.shape
+    dfdy = np.zeros((ni * nz, ni * nz))
+    indx = [np.arange(j, j + ni * nz, ni) for j in range(ni)]
+    for i in range(ni):
+        for m in range(ni):
+            dfdy[indx[i], indx[m]] = -(J[:, i, m])
+    return dfdy
+
+
+# ---------------------------------------------------------------------------
+# A4: diffusion right-hand side
+# ---------------------------------------------------------------------------
+
+def _drift(col, j_int, gl):
+    """-1/Hpi + ms g/(Navo kb Ti) + alpha/Ti (T[j+1]-T[j])/dzi at interface j_int with gravity gl."""
+    Ti, Hpi, Tco, dzi = col.Ti, col.Hpi, col.Tco, col.dzi
+    return -1. / Hpi[j_int] + col.ms * gl / (Navo * kb * Ti[j_int]) \
+        + col.alpha / Ti[j_int] * (Tco[j_int + 1] - Tco[j_int]) / dzi[j_int]
+
+
+def diffdf(col, y, use_moldiff=True) -> np.ndarray:
+    """op.py:1496-1597 (use_moldiff) or op.py:1438-1494; vz = 0, no flux BCs."""
+    nz, ni = y.shape
+    ysum = np.sum(y, axis=1)
+    dzi, Kzz, Dzz, g = col.dzi, col.Kzz, col.Dzz, col.g
+    A, B, C = np.zeros(nz), np.zeros(nz), np.zeros(nz)
+    A[0] = -1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0]
+    B[0] = 1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1]
+    A[nz - 1] = -1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1]
+    C[nz - 1] = 1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2]
+    j = np.arange(1, nz - 1)
+    dz_ave = 0.5 * (dzi[j - 1] + dzi[j])
+    A[j] = -1. / dz_ave * (Kzz[j] / dzi[j] * (ysum[j + 1] + ysum[j]) / 2. + Kzz[j - 1] / dzi[j - 1] * (ysum[j] + ysum[j - 1]) / 2.) / ysum[j]
+    B[j] = 1. / dz_ave * Kzz[j] / dzi[j] * (ysum[j + 1] + ysum[j]) / 2. / ysum[j + 1]
+    C[j] = 1. / dz_ave * Kzz[j - 1] / dzi[j - 1] * (ysum[j] + ysum[j - 1]) / 2. / ysum[j - 1]
+    if not use_moldiff:
+        diff = np.empty_like(y)
+        diff[0] = A[0] * y[0] + B[0] * y[1]
+        diff[1:nz - 1] = A[j, None] * y[j] + B[j, None] * y[j + 1] + C[j, None] * y[j - 1]
+        diff[nz - 1] = A[nz - 1] * y[nz - 1] + C[nz - 1] * y[nz - 2]
+        return diff
+    Ai = np.zeros((nz, ni)); Bi = np.zeros((nz, ni)); Ci = np.zeros((nz, ni))
+    d0 = _drift(col, 0, g[0])
+    Ai[0] = -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+    Bi[0] = 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+    dt_ = _drift(col, nz - 2, g[nz - 1])
+    Ai[nz - 1] = -1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1] \
+        - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
+    Ci[nz - 1] = 1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2] \
+        - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
+    for jj in range(1, nz - 1):
+        da = 0.5 * (dzi[jj - 1] + dzi[jj])
+        Ai[jj] = -1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj] + ysum[jj - 1]) / 2.) / ysum[jj]
+        Bi[jj] = 1. / da * Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. / ysum[jj + 1]
+        Ci[jj] = 1. / da * Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj] + ysum[jj - 1]) / 2. / ysum[jj - 1]
+        Ai[jj] += 1. / (2. * da) * (Dzz[jj] * _drift(col, jj, g[jj]) - Dzz[jj - 1] * _drift(col, jj - 1, g[jj]))
+        Bi[jj] += 1. / (2. * da) * Dzz[jj] * _drift(col, jj, g[jj + 1])
+        Ci[jj] += -1. / (2. * da) * Dzz[jj - 1] * _drift(col, jj - 1, g[jj - 1])
+    diff = np.empty_like(y)
+    diff[0] = (A[0] + Ai[0]) * y[0] + (B[0] + Bi[0]) * y[1]
+    mid = A[j, None] * y[j] + B[j, None] * y[j + 1] + C[j, None] * y[j - 1]
+    mid += Ai[j] * y[j] + Bi[j] * y[j + 1] + Ci[j] * y[j - 1]
+    diff[1:nz - 1] = mid
+    diff[nz - 1] = (A[nz - 1] + Ai[nz - 1]) * y[nz - 1] + (C[nz - 1] + Ci[nz - 1]) * y[nz - 2]
+    return diff
+
+
+# ---------------------------------------------------------------------------
+# A5: Ros2 left-hand side as blocks
+# ---------------------------------------------------------------------------
+
+def lhs_blocks(col, y, dt, J, use_moldiff=True):
+    """lhs = c0 I - J_chem - J_diff (op.py:1973-2042 / 2122-2166).
+
+    Returns D (nz, ni, ni) diagonal blocks, Lsub (nz, ni) with Lsub[j] the
+    diagonal of block (j, j-1) and Usup (nz, ni) with Usup[j] the diagonal
+    of block (j, j+1); Lsub[0] = Usup[nz-1] = 0.
+    """
+    nz, ni = y.shape
+    ysum = np.sum(y, axis=1)
+    dzi, Kzz, Dzz, g = col.dzi, col.Kzz, col.Dzz, col.g
+    c0 = 1. / (R_ROS * dt)
+    D = -J.copy()
+    idx = np.arange(ni)
+    D[:, idx, idx] = c0 + D[:, idx, idx]                  # np.fill_diagonal(dfdy, c0 + diag)
+    L = np.zeros((nz, ni)); U = np.zeros((nz, ni))
+    for jj in range(1, nz - 1):
+        da = 0.5 * (dzi[jj - 1] + dzi[jj])
+        D[jj, idx, idx] -= -1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj]
+        U[jj] -= 1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1]))
+        L[jj] -= 1. / da * (Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1]))
+        if use_moldiff:
+            D[jj, idx, idx] -= -1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj] \
+                + 1. / (2. * da) * (Dzz[jj] * _drift(col, jj, g[jj]) - Dzz[jj - 1] * _drift(col, jj - 1, g[jj]))
+            U[jj] -= 1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1])) \
+                + 1. / (2. * da) * Dzz[jj] * _drift(col, jj, g[jj + 1])
+            L[jj] -= 1. / da * (Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1])) \
+                - 1. / (2. * da) * Dzz[jj - 1] * _drift(col, jj - 1, g[jj - 1])
+    D[0, idx, idx] -= -1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0])
+    if use_moldiff:
+        d0 = _drift(col, 0, g[0])
+        D[0, idx, idx] -= -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0]) \
+            + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+    U[0] -= 1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1])
+    if use_moldiff:
+        U[0] -= 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1]) \
+            + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+    t = nz - 1
+    D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t])
+    if use_moldiff:
+        dtop = _drift(col, nz - 2, g[t])
+        D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t]) \
+            - 1. / (dzi[-1]) * Dzz[-1] / 2. * dtop
+    L[t] -= 1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t - 1])
+    if use_moldiff:
+        L[t] -= 1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t - 1]) \
+            - 1. / (dzi[-1]) * Dzz[-1] / 2. * dtop
+    return D, L, U
+
+
+def lhs_dense(D, L, U):
+    """Dense (ni*nz)^2 matrix from blocks (small sizes and the CPU baseline)."""
+    nz, ni, _ = D.shape
+    N = ni * nz
+    a = np.zeros((N, N))
+    for j in range(nz):
+        s = slice(j * ni, (j + 1) * ni)
+        a[s, s] = D[j]
+        if j > 0:
+            a[np.arange(j * ni, (j + 1) * ni), np.arange((j - 1) * ni, j * ni)] = L[j]
+        if j < nz - 1:
+            a[np.arange(j * ni, (j + 1) * ni), np.arange((j + 1) * ni, (j + 2) * ni)] = U[j]
+    return a
+
+
+# ---------------------------------------------------------------------------
+# A6/A7: band storage and the banded solve
+# ---------------------------------------------------------------------------
+
+def store_bandM(a, nb, nn):
+    """op.py:2833-2858: dense -> LAPACK diagonal-ordered band storage."""
+    bw = 2 * nb - 1
+    ab = np.zeros((2 * bw + 1, nb * nn))
+    for i in range(0, 2 * nb):
+        ab[-(2 * nb + i):, i] = a[0:2 * nb + i, i]
+    for i in range(2 * nb, nn * nb - 2 * nb):
+        ab[:, i] = a[(i - 2 * nb + 1):(i - 2 * nb + 1) + (2 * bw + 1), i]
+    for ne, i in enumerate(range(nn * nb - 2 * nb, nn * nb)):
+        ab[:(2 * bw + 1 - ne), i] = a[-(2 * bw + 1 - ne):, i]
+    return ab, bw
+
+
+def band_from_blocks(D, L, U):
+    """The same band storage filled straight from the blocks: ab[bw+i-j, j] = a[i, j]."""
+    nz, ni, _ = D.shape
+    bw = 2 * ni - 1
+    N = ni * nz
+    ab = np.zeros((2 * bw + 1, N))
+    rows = np.arange(ni)
+    for j in range(nz):
+        c = j * ni + rows                      # global column index
+        for i in range(ni):
+            r = j * ni + i
+            ab[bw + r - c, c] = D[j, i, :]
+        if j > 0:
+            r = j * ni + rows
+            cc = (j - 1) * ni + rows
+            ab[bw + r - cc, cc] = L[j]
+        if j < nz - 1:
+            r = j * ni + rows
+            cc = (j + 1) * ni + rows
+            ab[bw + r - cc, cc] = U[j]
+    return ab, bw
+
+
+# ---------------------------------------------------------------------------
+# A8-A11: solver state machine and integration loop
+# ---------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class State:
+    y: np.ndarray
+    ymix: np.ndarray
+    k: np.ndarray
+    dt: float
+    atom_ini: np.ndarray
+    t: float = 0.0
+    count: int = 0
+    delta: float = 0.0
+    y_prev: Optional[np.ndarray] = None
+    atom_sum: Optional[np.ndarray] = None
+    atom_loss: Optional[np.ndarray] = None
+    atom_loss_prev: Optional[np.ndarray] = None
+    small_y: float = 0.0
+    nega_y: float = 0.0
+    nega_count: int = 0
+    loss_count: int = 0
+    delta_count: int = 0
+    dy: float = 1.0
+    dydt: float = 1.0
+    longdy: float = 1.0
+    longdydt: float = 1.0
+    aflux_change: float = 0.0
+    end_case: int = 0
+    y_time: List[np.ndarray] = dataclasses.field(default_factory=list)
+    t_time: List[float] = dataclasses.field(default_factory=list)
+    k1: Optional[np.ndarray] = None
+    k2: Optional[np.ndarray] = None
+    attempts: int = 0
+
+
+def atom_sums(y, atom_mat):
+    """loss(): sum_i compo[i, a] * y[:, i] over all layers (op.py:2472-2487)."""
+    ni = y.shape[1]
+    return np.array([np.sum([atom_mat[i, a] * y[:, i] for i in range(ni)])
+                     for a in range(atom_mat.shape[1])])
+
+
+def new_state(net, y, k, cfg: Config) -> State:
+    """Initial state: ymix from y, element baseline atom_ini (build_atm.py:283-291)."""
+    atom_mat = net.atom_matrix()
+    st = State(y=y.copy(), ymix=y / np.sum(y, axis=1)[:, None], k=k, dt=cfg.dttry,
+               atom_ini=atom_sums(y, atom_mat))
+    st.atom_sum = st.atom_ini.copy()
+    st.atom_loss = np.zeros_like(st.atom_ini)
+    st.atom_loss_prev = np.zeros_like(st.atom_ini)
+    st.y_prev = st.y.copy()
+    return st
+
+
+def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False):
+    """One Ros2 attempt (op.py:2860-3007).  Mutates st.y, st.ymix, st.delta."""
+    tables = tables or nw.build_tables(net)
+    y, ymix, h, k = st.y, st.ymix, st.dt, st.k
+    M = col.M
+    r = R_ROS
+    df = chemdf(net, y, M, k, tables).flatten() + diffdf(col, y, cfg.use_moldiff).flatten()
+    J = jac_blocks(net, y, M, k, tables)
+    D, L, U = lhs_blocks(col, y, h, J, cfg.use_moldiff)
+    if dense_lhs:
+        # the reference's cost profile: dense lhs, then the column-loop repack
+        lhs = neg_symjac_dense(J)
+        Dd, Ld, Ud = lhs_blocks(col, y, h, np.zeros_like(J), cfg.use_moldiff)
+        lhs += lhs_dense(Dd, Ld, Ud)
+        lhs_b, bw = store_bandM(lhs, net.ni, y.shape[0])
+        del lhs
+    else:
+        lhs_b, bw = band_from_blocks(D, L, U)
+    k1_flat = scipy.linalg.solve_banded((bw, bw), lhs_b, df)
+    k1 = k1_flat.reshape(y.shape)
+    yk2 = y + k1 / r
+    df = chemdf(net, yk2, M, k, tables).flatten() + diffdf(col, yk2, cfg.use_moldiff).flatten()
+    rhs = df - 2. / (r * h) * k1_flat
+    k2 = scipy.linalg.solve_banded((bw, bw), lhs_b, rhs).reshape(y.shape)
+    sol = y + 3. / (2. * r) * k1 + 1 / (2. * r) * k2
+    delta = np.abs(sol - yk2)
+    delta[ymix < cfg.mtol] = 0
+    delta[sol < cfg.atol] = 0
+    pos = sol > 0
+    delta = np.amax(delta[pos] / sol[pos])
+    st.y = sol
+    st.ymix = st.y / np.vstack(np.sum(st.y, axis=1))
+    st.delta = delta
+    st.k1, st.k2 = k1, k2
+    st.attempts += 1
+    return st
+
+
+def clip(st: State, atom_mat, cfg: Config):
+    """op.py:2447-2470 (loss at 2472-2487)."""
+    y, ymix = st.y, st.ymix.copy()
+    st.small_y += np.abs(np.sum(y[np.logical_and(y < cfg.pos_cut, y >= 0)]))
+    st.nega_y += np.abs(np.sum(y[np.logical_and(y > cfg.nega_cut, y <= 0)]))
+    y[np.logical_and(y < cfg.pos_cut, y >= cfg.nega_cut)] = 0.
+    y[np.logical_and(ymix < cfg.mtol, y < 0)] = 0.
+    st.atom_sum = atom_sums(y, atom_mat)
+    st.atom_loss = (st.atom_sum - st.atom_ini) / st.atom_ini
+    st.y, st.ymix = y, y / np.vstack(np.sum(y, axis=1))
+    return st
+
+
+def step_ok(st: State, cfg: Config) -> bool:
+    """op.py:2489-2493 (rtol and loss_eps bound at import: cfg values)."""
+    return bool(np.all(st.y >= 0) and np.amax(np.abs(st.atom_loss - st.atom_loss_prev)) < cfg.loss_eps
+                and st.delta <= cfg.rtol)
+
+
+def step_reject(st: State, cfg: Config) -> bool:
+    """op.py:2495-2534.  Returns True when the solver gives up at dt < dt_min."""
+    if st.delta > cfg.rtol:
+        st.delta_count += 1
+    elif np.any(st.y < 0):
+        st.nega_count += 1
+    else:
+        st.loss_count += 1
+    st.y = st.y_prev                     # reset_y: ymix stays stale (op.py:2524-2534)
+    st.dt *= cfg.dt_var_min
+    if st.dt < cfg.dt_min:
+        st.dt = cfg.dt_min
+        st.y[st.y < 0] = 0.
+        return True
+    return False
+
+
+def one_step(net, col, st: State, cfg: Config, tables=None, max_attempts=10 ** 6, dense_lhs=False):
+    """op.py:3091-3103."""
+    atom_mat = net.atom_matrix()
+    for _ in range(max_attempts):
+        ros2_solver(net, col, st, cfg, tables, dense_lhs=dense_lhs)
+        clip(st, atom_mat, cfg)
+        if step_ok(st, cfg):
+            break
+        if step_reject(st, cfg):
+            break
+    return st
+
+
+def step_size(st: State, cfg: Config):
+    """op.py:3105-3125."""
+    h = st.dt
+    delta = st.delta
+    rtol = cfg.rtol
+    if delta == 0:
+        delta = 0.01 * rtol
+    h_factor = 0.9 * (rtol / delta) ** 0.5
+    h_factor = np.maximum(h_factor, cfg.dt_var_min)
+    h_factor = np.minimum(h_factor, cfg.dt_var_max)
+    h *= h_factor
+    h = np.maximum(h, cfg.dt_min)
+    h = np.minimum(h, cfg.dt_max)
+    st.dt = float(h)
+    return st
+
+
+def update_mu_dz(col, st: State):
+    """op.py:944-984 (mean_mass build_atm.py:515-520); mutates col."""
+    nz = col.nz
+    mu = np.zeros(nz)
+    for i in range(st.ymix.shape[1]):
+        mu += col.ms[i] * st.ymix[:, i]
+    col.mu = mu
+    Tco, pico = col.Tco, col.pico
+    pref = col.pref_indx
+    for i in range(pref, nz):
+        if i == pref:
+            col.g[i] = col.gs
+            col.Hp[i] = kb * Tco[i] / (col.mu[i] / Navo * col.gs)
+        else:
+            col.g[i] = col.gs * (col.Rp / (col.Rp + col.zco[i])) ** 2
+            col.Hp[i] = kb * Tco[i] / (col.mu[i] / Navo * col.g[i])
+        col.dz[i] = col.Hp[i] * np.log(pico[i] / pico[i + 1])
+        col.zco[i + 1] = col.zco[i] + col.dz[i]
+    if pref != 0:
+        for i in range(pref - 1, -1, -1):
+            col.g[i] = col.gs * (col.Rp / (col.Rp + col.zco[i + 1])) ** 2
+            col.Hp[i] = kb * Tco[i] / (col.mu[i] / Navo * col.g[i])
+            col.dz[i] = col.Hp[i] * np.log(pico[i] / pico[i + 1])
+            col.zco[i] = col.zco[i + 1] - col.dz[i]
+    col.dzi = (0.5 * (col.dz + np.roll(col.dz, 1)))[1:]
+    col.Ti = (0.5 * (Tco + np.roll(Tco, -1)))[:-1]
+    col.Hpi = (0.5 * (col.Hp + np.roll(col.Hp, -1)))[:-1]
+    return col
+
+
+def f_dy(st: State, cfg: Config):
+    """op.py:1003-1015."""
+    if st.count == 0:
+        st.dy, st.dydt = 1., 1.
+        return st
+    y, ymix, y_prev = st.y, st.ymix, st.y_prev
+    dy = np.abs(y - y_prev)
+    dy[ymix < cfg.mtol] = 0
+    dy[y < cfg.atol] = 0
+    dy = np.amax(dy[y > 0] / y[y > 0])
+    st.dy, st.dydt = dy, dy / st.dt
+    return st
+
+
+def conv(col, st: State, cfg: Config, species=None) -> bool:
+    """op.py:1018-1063 (aflux_change stays 0 without radiative transfer)."""
+    y, ymix, y_time, t_time = st.y.copy(), st.ymix.copy(), st.y_time, np.asarray(st.t_time)
+    count = st.count
+    slope_min = min(np.amin(col.Kzz / (0.1 * col.Hp[:-1]) ** 2), 1.e-8)
+    slope_min = max(slope_min, 1.e-10)
+    indx = np.abs(t_time - st.t * cfg.st_factor).argmin()
+    if indx == count - 1:
+        indx -= 1
+    indx = max(count - cfg.conv_step, indx)
+    longdy = np.abs((y_time[count - 1] - y_time[indx]) / np.vstack(col.M))
+    longdy[ymix < cfg.mtol_conv] = 0
+    longdy[y < cfg.atol] = 0
+    for sp in cfg.conver_ignore:
+        longdy[:, species.index(sp)] = 0
+    longdy = np.amax(longdy[ymix > 0] / ymix[ymix > 0])
+    longdydt = longdy / (t_time[-1] - t_time[indx])
+    st.longdy, st.longdydt = longdy, longdydt
+    return bool((longdy < cfg.yconv_cri and longdydt < cfg.slope_cri or
+                 longdy < cfg.yconv_min and longdydt < slope_min) and st.aflux_change < cfg.flux_cri)
+
+
+def stop(col, st: State, cfg: Config, species=None) -> bool:
+    """op.py:1065-1085."""
+    if st.t > cfg.trun_min and st.count > cfg.count_min and conv(col, st, cfg, species):
+        st.end_case = 1
+        return True
+    if st.t > cfg.runtime:
+        st.end_case = 2
+        return True
+    if st.count > cfg.count_max:
+        st.end_case = 3
+        return True
+    return False
+
+
+def integrate(net, col, st: State, cfg: Config, max_steps=None, tables=None, update_mu=True,
+              keep_history=True):
+    """Integration.__call__ (op.py:808-936) without photo RT, condensation or plots.
+
+    ``max_steps`` bounds the number of accepted steps taken in this call (the
+    reference has no such knob; it stops only through ``stop``).
+    """
+    tables = tables or nw.build_tables(net)
+    taken = 0
+    while not stop(col, st, cfg, net.species):
+        if max_steps is not None and taken >= max_steps:
+            break
+        st.y_prev = np.copy(st.y)                      # backup, op.py:938-942
+        st.atom_loss_prev = st.atom_loss.copy()
+        one_step(net, col, st, cfg, tables)
+        if update_mu and st.count % cfg.update_frq == 0:
+            update_mu_dz(col, st)
+        st.y = np.vstack(col.M) * st.ymix              # op.py:909-914
+        f_dy(st, cfg)
+        st.t += st.dt                                  # save_step, op.py:1087-1105
+        st.count += 1
+        if keep_history:
+            st.y_time.append(st.y)
+        st.t_time.append(st.t)
+        step_size(st, cfg)
+        taken += 1
+    return st

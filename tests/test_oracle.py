@@ -1,0 +1,396 @@
+"""Pins the CPU oracle (oracle/ros2_oracle.py).
+
+Two kinds of pins:
+
+* golden vectors written by the reference itself (tests/golden/make_golden.py)
+  for the generated chem_funs network (65 species): K_eq, chemdf, the
+  Jacobian, diffdf, the lhs blocks, one solver attempt, one_step+step_size and
+  a 61-step Integration trajectory with rejections, update_mu_dz and conv;
+* closed forms and invariants: the Ros2 stability function on y' = lambda y,
+  complex-step derivatives of chemdf, element conservation, detailed-balance
+  fixed points, telescoping of the eddy-diffusion flux, band storage against a
+  dense solve, and hand-evaluated rate expressions.
+
+Each pin is chosen so that a dropped term, a wrong sign or index, or a
+transposed operand in the restatement fails it.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from _golden_util import load, network_from, column_from, traj_config, rel_err
+from oracle import ros2_oracle as O
+from vulcan_amd import network as nw
+from vulcan_amd import workloads as W
+from vulcan_amd.config import Config
+
+
+@pytest.fixture(scope='module')
+def gd():
+    d = load()
+    net = network_from(d)
+    return d, net, column_from(d), nw.build_tables(net)
+
+
+# --------------------------------------------------------------------------
+# golden vectors from the reference
+# --------------------------------------------------------------------------
+
+def test_gibbs_keq_bitwise(gd):
+    d, net, col, tb = gd
+    with np.errstate(all='ignore'):
+        K = np.array([O.gibbs_K(net, p, d['Tg']) for p in range(net.npairs)])
+    assert np.array_equal(K, d['Kg'])
+
+
+def test_chemdf_bitwise(gd):
+    d, net, col, tb = gd
+    f = O.chemdf(net, d['y0'], d['M'], d['k'], tb)
+    assert np.array_equal(f, d['chemdf'])
+
+
+def test_jacobian_blocks(gd):
+    d, net, col, tb = gd
+    J = O.jac_blocks(net, d['y0'], d['M'], d['k'], tb)
+    nj = d['negjac']
+    scale = np.maximum(np.abs(nj).max(axis=2, keepdims=True), 1e-300)
+    assert np.max(np.abs(-J - nj) / scale) < 2e-15
+    # rows of species that take part in no reaction are zero in both
+    assert np.array_equal(np.abs(J).max(axis=2) == 0, np.abs(nj).max(axis=2) == 0)
+
+
+@pytest.mark.parametrize('mol,key', [(True, 'diffdf_mol'), (False, 'diffdf_nomol')])
+def test_diffdf_bitwise(gd, mol, key):
+    d, net, col, tb = gd
+    assert np.array_equal(O.diffdf(col, d['y0'], mol), d[key])
+
+
+@pytest.mark.parametrize('mol,key', [(True, 'tot'), (False, 'nomol')])
+def test_lhs_blocks(gd, mol, key):
+    d, net, col, tb = gd
+    y = d['y0']
+    J = O.jac_blocks(net, y, d['M'], d['k'], tb)
+    D, L, U = O.lhs_blocks(col, y, float(d['lhs_dt']), J, mol)
+    assert np.array_equal(L, d['lhs_%s_L' % key])
+    assert np.array_equal(U, d['lhs_%s_U' % key])
+    # with the reference's own Jacobian blocks the diagonal blocks are bitwise equal
+    D2, _, _ = O.lhs_blocks(col, y, float(d['lhs_dt']), -d['negjac'], mol)
+    assert np.array_equal(D2, d['lhs_%s_D' % key])
+    assert np.max(np.abs(D - d['lhs_%s_D' % key]) / np.abs(d['lhs_%s_D' % key]).max(axis=2, keepdims=True)) < 2e-15
+
+
+def _state(d, net, y, dt, cfg):
+    st = O.new_state(net, y.copy(), d['k'], cfg)
+    st.dt = dt
+    return st
+
+
+def test_solver_attempt(gd):
+    d, net, col, tb = gd
+    cfg = Config()
+    st = _state(d, net, d['y0'], float(d['solver_dt']), cfg)
+    assert np.array_equal(st.ymix, d['solver_ymix_in'])
+    O.ros2_solver(net, col, st, cfg, tb)
+    assert rel_err(st.y, d['solver_sol'], floor=1e-3 * d['M'].min()) < 1e-10
+    assert abs(st.delta - float(d['solver_delta'])) <= 1e-10 * float(d['solver_delta'])
+
+
+def test_solver_attempt_with_reference_jacobian_is_bitwise(gd, monkeypatch):
+    """With the reference's own Jacobian the attempt reproduces it exactly:
+    everything else (rhs, lhs assembly, band storage, solve) is bit-identical."""
+    d, net, col, tb = gd
+    cfg = Config()
+    monkeypatch.setattr(O, 'jac_blocks', lambda *a, **k: -d['negjac'])
+    st = _state(d, net, d['y0'], float(d['solver_dt']), cfg)
+    O.ros2_solver(net, col, st, cfg, tb)
+    assert np.array_equal(st.y, d['solver_sol'])
+    assert np.array_equal(st.ymix, d['solver_ymix'])
+    assert st.delta == float(d['solver_delta'])
+
+
+def test_one_step_with_rejections(gd, monkeypatch):
+    d, net, col, tb = gd
+    cfg = Config()
+    monkeypatch.setattr(O, 'jac_blocks', lambda *a, **k: -d['negjac'])
+    st = _state(d, net, d['y0'], float(d['onestep_dt_in']), cfg)
+    st.y_prev = st.y.copy()
+    O.one_step(net, col, st, cfg, tb)
+    assert np.array_equal([st.delta_count, st.nega_count, st.loss_count], d['onestep_counts'])
+    assert np.array_equal(st.y, d['onestep_y'])
+    assert st.delta == float(d['onestep_delta'])
+    assert rel_err(st.atom_loss, d['onestep_atom_loss'], floor=1e-15) < 1e-6
+    O.step_size(st, cfg)
+    assert st.dt == float(d['onestep_dt_out'])
+
+
+def test_trajectory(gd, monkeypatch):
+    """61 accepted steps: rejections, update_mu_dz every 10 steps, conv."""
+    d, net, col, tb = gd
+    cfg = traj_config(d)
+    for key in ('g', 'Hp', 'dz', 'zco', 'dzi', 'Ti', 'Hpi'):
+        setattr(col, key, d['trajg0_' + key].copy())
+    jac_ref = O.jac_blocks
+
+    st = O.new_state(net, d['y0'].copy(), d['k'], cfg)
+    O.integrate(net, col, st, cfg, tables=tb)
+    ty = d['traj_y']
+    fin = d['traj_final']
+    assert st.count == int(fin[5]) and st.end_case == int(fin[4])
+    assert [st.delta_count, st.nega_count, st.loss_count] == [int(x) for x in fin[6:9]]
+    assert rel_err(np.asarray(st.t_time), d['traj_t']) < 1e-9
+    ys = np.stack(st.y_time)
+    assert ys.shape == ty.shape
+    assert rel_err(ys, ty, floor=1e-6 * d['M'].min()) < 1e-6
+    assert rel_err(st.dt, fin[0]) < 1e-6
+    assert rel_err(st.longdy, fin[2]) < 1e-6
+    assert rel_err(col.g, d['traj_geom'][0]) < 1e-12
+    assert rel_err(col.dzi, d['traj_geom'][2][:-1]) < 1e-12
+    assert jac_ref is O.jac_blocks
+
+
+# --------------------------------------------------------------------------
+# closed forms and invariants
+# --------------------------------------------------------------------------
+
+def test_ros2_stability_function():
+    """For y' = lambda*y one Ros2 step multiplies y by
+    R(z) = (1 - (1+sqrt2) z) / (1 - (1+1/sqrt2) z)^2 (derived from op.py:2890-2932),
+    and yk2/y = 1 + z/(1 - gamma z)."""
+    g = 1 + 1 / np.sqrt(2)
+    for lam, h in [(-3.0, 0.1), (-1e4, 1e-2), (2.0, 0.05), (-1e8, 1.0)]:
+        y = 1.7
+        c0 = 1 / (g * h)
+        k1 = (lam * y) / (c0 - lam)
+        yk2 = y + k1 / g
+        k2 = (lam * yk2 - 2 / (g * h) * k1) / (c0 - lam)
+        sol = y + 3 / (2 * g) * k1 + 1 / (2 * g) * k2
+        z = h * lam
+        R = (1 - (1 + np.sqrt(2)) * z) / (1 - g * z) ** 2
+        assert abs(sol / y - R) < 1e-12 * max(1, abs(R))
+        assert abs(yk2 / y - (1 + z / (1 - g * z))) < 1e-12
+
+
+def test_ros2_matches_stability_function_on_linear_diffusion():
+    """With chemistry off (k = 0) the column is linear: one oracle attempt
+    must equal R(h*A) y for the assembled diffusion operator A."""
+    c = W.small_case(nz=12)
+    net, col = c.net, c.col
+    cfg = Config()
+    k = np.zeros((col.nz, net.nr))
+    y = c.y0 * (1 + 0.3 * np.random.default_rng(3).random(c.y0.shape))
+    st = O.new_state(net, y, k, cfg)
+    st.dt = 1e3
+    ysum = y.sum(axis=1)
+    D, L, U = O.lhs_blocks(col, y, st.dt, np.zeros((col.nz, net.ni, net.ni)), True)
+    A = -(O.lhs_dense(D, L, U) - np.eye(y.size) / (O.R_ROS * st.dt))
+    O.ros2_solver(net, col, st, cfg)
+    # frozen-ysum operator: diffdf(y) is linear in y at fixed ysum
+    assert np.allclose(A @ y.ravel(), O.diffdf(col, y, True).ravel(), rtol=1e-10, atol=1e-10 * np.abs(A @ y.ravel()).max())
+    g = O.R_ROS
+    Mx = np.eye(y.size) - g * st.dt * A
+    k1 = np.linalg.solve(Mx, g * st.dt * A @ y.ravel())   # (c0 I - A)^-1 A y, c0 = 1/(g h)
+    assert np.allclose(st.k1.ravel(), k1, rtol=1e-8, atol=1e-8 * np.abs(k1).max())
+    # (the second stage is not linear: diffdf rescales by ysum(yk2), so the
+    # full step is pinned by the scalar stability-function test above)
+    del ysum
+
+
+def _complex_chemdf(net, y, M, k, tb):
+    v = np.empty((y.shape[0], net.npairs), dtype=complex)
+    for p in range(net.npairs):
+        a = k[:, 2 * p].astype(complex)
+        for t in net.reac[p]:
+            if t != nw.TERM_NONE:
+                a = a * (M if t == nw.TERM_M else y[:, t])
+        b = k[:, 2 * p + 1].astype(complex)
+        for t in net.prod[p]:
+            if t != nw.TERM_NONE:
+                b = b * (M if t == nw.TERM_M else y[:, t])
+        v[:, p] = a - b
+    out = np.zeros(y.shape, dtype=complex)
+    for i in range(net.ni):
+        for e in range(tb.chem_ptr[i], tb.chem_ptr[i + 1]):
+            out[:, i] += tb.chem_sign[e] * v[:, tb.chem_pair[e]]
+    return out
+
+
+def test_jacobian_complex_step():
+    """chemdf is polynomial in y, so Im f(y + i h e_m)/h is exact to rounding."""
+    c = W.small_case(nz=4)
+    net = nw.synthetic_network(24, 40, 6, 3, seed=77)
+    tb = nw.build_tables(net)
+    rng = np.random.default_rng(5)
+    y = 10.0 ** rng.uniform(8, 16, (4, net.ni))
+    M = y.sum(axis=1)
+    k = 10.0 ** rng.uniform(-30, -10, (4, net.nr))
+    J = O.jac_blocks(net, y, M, k, tb)
+    for m in range(net.ni):
+        h = 1e-30 * y[:, m]
+        yc = y.astype(complex)
+        yc[:, m] += 1j * h
+        col = (_complex_chemdf(net, yc, M, k, tb).imag / h[:, None])
+        scale = np.abs(J).max() + 1e-300
+        assert np.max(np.abs(col - J[:, :, m])) <= 1e-13 * scale
+
+
+def test_chemdf_conserves_elements_and_scales():
+    net = W.small_network()
+    tb = nw.build_tables(net)
+    rng = np.random.default_rng(11)
+    y = 10.0 ** rng.uniform(6, 16, (5, net.ni))
+    M = y.sum(axis=1)
+    k = 10.0 ** rng.uniform(-16, -9, (5, net.nr))
+    f = O.chemdf(net, y, M, k, tb)
+    E = net.atom_matrix()
+    terms = np.abs(O.pair_rates(net, y, M, k)).max(axis=1)
+    assert np.all(np.abs(f @ E).max(axis=1) <= 1e-12 * terms * 4)
+    # all pairs bimolecular without M: chemdf(a y) = a^2 chemdf(y)
+    f2 = O.chemdf(net, 3.0 * y, M, k, tb)
+    assert np.allclose(f2, 9.0 * f, rtol=1e-12, atol=1e-12 * np.abs(f).max())
+
+
+def test_chemdf_detailed_balance_fixed_point():
+    """k_r = k_f prod(y_reac)/prod(y_prod) makes every pair rate, and chemdf, vanish."""
+    net = W.full_network()
+    tb = nw.build_tables(net)
+    rng = np.random.default_rng(12)
+    y = 10.0 ** rng.uniform(8, 14, (3, net.ni))
+    M = 10.0 ** rng.uniform(17, 19, 3)
+    k = np.zeros((3, net.nr))
+    for p in range(net.npairs):
+        kf = 10.0 ** rng.uniform(-12, -9)
+        pr = np.ones(3)
+        for t in net.reac[p]:
+            if t != nw.TERM_NONE:
+                pr = pr * (M if t == nw.TERM_M else y[:, t])
+        pp = np.ones(3)
+        for t in net.prod[p]:
+            if t != nw.TERM_NONE:
+                pp = pp * (M if t == nw.TERM_M else y[:, t])
+        k[:, 2 * p] = kf
+        k[:, 2 * p + 1] = kf * pr / pp
+    v = O.pair_rates(net, y, M, k)
+    f = O.chemdf(net, y, M, k, tb)
+    scale = (k[:, 0::2] * 1).max() * y.max() ** 2 * M.max()
+    assert np.abs(f).max() <= 1e-13 * scale
+    assert np.abs(v).max() <= 1e-13 * scale
+
+
+def test_eddy_diffusion_telescopes_and_keeps_well_mixed():
+    """Kzz part in flux form: sum_j w_j diff[j, i] = 0 with w = interface-averaged
+    dz (op.py:1534-1539, 1560-1564); a well-mixed column is a fixed point."""
+    c = W.small_case(nz=20, use_moldiff=False)
+    col = c.col
+    rng = np.random.default_rng(13)
+    y = c.y0 * (1 + rng.random(c.y0.shape))
+    diff = O.diffdf(col, y, use_moldiff=False)
+    nz = col.nz
+    w = np.empty(nz)
+    w[0] = col.dzi[0]
+    w[nz - 1] = col.dzi[nz - 2]
+    w[1:nz - 1] = 0.5 * (col.dzi[:-1] + col.dzi[1:])
+    flux_scale = (np.abs(diff) * w[:, None]).sum(axis=0)
+    assert np.all(np.abs(w @ diff) <= 1e-12 * flux_scale)
+    # well mixed: y_i proportional to ysum
+    x = rng.random(c.net.ni)
+    x /= x.sum()
+    ywm = col.M[:, None] * x[None, :]
+    dwm = O.diffdf(col, ywm, use_moldiff=False)
+    assert np.abs(dwm).max() <= 1e-12 * np.abs(diff).max()
+
+
+def test_store_bandM_and_band_solve():
+    rng = np.random.default_rng(14)
+    nz, ni = 7, 5
+    D = rng.standard_normal((nz, ni, ni)) + 6 * np.eye(ni)
+    L = rng.standard_normal((nz, ni)); L[0] = 0
+    U = rng.standard_normal((nz, ni)); U[-1] = 0
+    a = O.lhs_dense(D, L, U)
+    ab1, bw1 = O.store_bandM(a, ni, nz)
+    ab2, bw2 = O.band_from_blocks(D, L, U)
+    assert bw1 == bw2 == 2 * ni - 1
+    assert np.array_equal(ab1, ab2)
+    # scipy's documented band layout: ab[u + i - j, j] = a[i, j]
+    for i in range(a.shape[0]):
+        for j in range(max(0, i - bw1), min(a.shape[0], i + bw1 + 1)):
+            assert ab1[bw1 + i - j, j] == a[i, j]
+    b = rng.standard_normal(nz * ni)
+    x = scipy.linalg.solve_banded((bw1, bw1), ab1, b)
+    assert np.allclose(x, np.linalg.solve(a, b), rtol=1e-12, atol=1e-12)
+
+
+def test_rate_forms_hand_values():
+    net = nw.Network(species=['A', 'B', 'C'], elements=['H'], compo=np.array([[1], [1], [2]], np.int32),
+                     mass=np.array([1., 1., 2.]), nasa9=np.zeros((3, 20)),
+                     reac=np.array([[0, 1, -2, -2], [0, 1, -1, -2], [0, 1, -1, -2], [2, -2, -2, -2]], np.int32),
+                     prod=np.array([[2, -2, -2, -2], [2, -1, -2, -2], [2, -1, -2, -2], [0, 1, -2, -2]], np.int32),
+                     kind=np.array([nw.KIND_ARRH, nw.KIND_LIND, nw.KIND_TROE, nw.KIND_PHOTO], np.int32),
+                     par=np.zeros((4, nw.NPAR)), rev=np.array([0, 0, 0, 0], np.int32))
+    net.par[0, :3] = (2e-11, 0.5, 1500.)
+    net.par[1, :6] = (3e-30, -1.0, 0., 2e-11, 0.2, 100.)
+    net.par[2, :7] = (3e-30, -1.0, 0., 2e-11, 0.2, 100., 0.6)
+    net.par[3, nw.P_J0] = 2e-4
+    T = np.array([700.])
+    M = np.array([2e18])
+    tau = np.array([0.3])
+    k = O.rates(net, T, M, tau)
+    assert np.isclose(k[0, 0], 2e-11 * 700 ** 0.5 * np.exp(-1500 / 700), rtol=1e-15)
+    k0 = 3e-30 / 700.
+    kinf = 2e-11 * 700 ** 0.2 * np.exp(-100 / 700)
+    lind = k0 / (1 + k0 * 2e18 / kinf)
+    assert np.isclose(k[0, 2], lind, rtol=1e-15)
+    # Troe-like broadening with natural logs, op.py:203-207
+    N = 0.75 - 1.27 * np.log(0.6)
+    ff = np.exp(np.log(0.6) / (1 + (np.log(k0 * 2e18 / kinf) / N) ** 2))
+    assert np.isclose(k[0, 4], lind * ff, rtol=1e-14)
+    assert np.isclose(k[0, 6], 2e-4 * np.exp(-0.3), rtol=1e-15)
+    assert np.all(k[0, 1::2] == 0)
+
+
+def test_detailed_balance_of_reverse_rates():
+    """k_f/k_r = K_eq with the (kT/P0)^dn factor for a termolecular pair."""
+    net = W.full_network()
+    c = W.full_case(nz=5)
+    k = O.rates(net, c.col.Tco, c.col.M, c.col.tau)
+    for p in range(net.npairs):
+        if not net.rev[p]:
+            assert np.all(k[:, 2 * p + 1] == 0)
+            continue
+        K = O.gibbs_K(net, p, c.col.Tco)
+        assert np.allclose(k[:, 2 * p] / k[:, 2 * p + 1], K, rtol=1e-13)
+    p_tb = int(np.nonzero(net.kind == nw.KIND_TROE)[0][0])
+    T = 1234.
+    g = [O.g_RT(np.array([T]), net.nasa9[s])[0] for s in range(net.ni)]
+    a, b, ab = net.reac[p_tb, 0], net.reac[p_tb, 1], net.prod[p_tb, 0]
+    Kexp = np.exp(-(g[ab] - g[a] - g[b])) * (O.CORR * T) ** 1
+    assert np.isclose(O.gibbs_K(net, p_tb, np.array([T]))[0], Kexp, rtol=1e-12)
+
+
+def test_step_size_rules():
+    cfg = Config()
+    st = O.State(y=np.ones((2, 2)), ymix=np.ones((2, 2)), k=None, dt=1.0, atom_ini=np.ones(1))
+    st.delta = 0.0
+    O.step_size(st, cfg)
+    assert st.dt == 2.0              # delta = 0 -> 0.01 rtol -> factor 9 -> capped at 2
+    st.delta = cfg.rtol * 4
+    O.step_size(st, cfg)
+    assert st.dt == 1.0              # factor 0.45 -> floored at 0.5
+    st.dt = 1e-20
+    st.delta = cfg.rtol
+    O.step_size(st, cfg)
+    assert st.dt == cfg.dt_min
+
+
+def test_reject_branches_and_give_up():
+    cfg = Config()
+    st = O.State(y=np.array([[1.0, -2.0]]), ymix=np.array([[0.5, 0.5]]), k=None, dt=1e-13,
+                 atom_ini=np.ones(1))
+    st.y_prev = np.array([[1.0, -0.5]])
+    st.delta = 0.1
+    assert O.step_reject(st, cfg) is False and st.nega_count == 1 and st.dt == 0.5e-13
+    st.y = np.array([[1.0, 1.0]]); st.delta = 1.0
+    assert O.step_reject(st, cfg) is False and st.delta_count == 1
+    st.y = np.array([[1.0, 1.0]]); st.delta = 0.1
+    st.dt = 1.5e-14
+    assert O.step_reject(st, cfg) is True and st.loss_count == 1
+    assert st.dt == cfg.dt_min and np.all(st.y >= 0)

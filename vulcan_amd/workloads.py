@@ -1,0 +1,114 @@
+"""Seeded synthetic workloads.
+
+* ``small``    one column, nz=60, ni=30, 150 reversible bimolecular pairs,
+               T linear 2000 K (bottom) -> 500 K (top), P 1e3 -> 1e-2 bar,
+               Kzz = 1e7 cm2/s, no photolysis, dt0 = 1e-10 s.
+* ``full``     one column, nz=150, ni=93, 600 pairs (15 % termolecular with
+               Troe falloff) plus 20 photolysis channels with
+               J = J0 exp(-tau), tau log-spaced 1e-4 (top) -> 1e2 (bottom);
+               Kzz log-linear 1e6 (bottom) -> 1e10 (top) cm2/s.
+* ``batched``  256 columns sharing the ``full`` network; nz uniform on
+               [80, 200]; isothermal-plus-gradient T with top and bottom T
+               uniform on [400, 2500] K; constant Kzz log-uniform on
+               [1e5, 1e11] cm2/s.
+* ``mixed``    512 columns over three networks (ni, pairs) = (40, 200),
+               (93, 600), (150, 1100; three extra elements); nz uniform on
+               [100, 150].
+
+Initial mixing ratios are log-uniform on [1e-12, 1e-1], normalised to 1, and
+``y = n_0 * ymix``.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+
+from .atmosphere import Column, build_column, initial_mixing
+from .config import Config
+from .network import Network, synthetic_network
+
+SEED_SMALL, SEED_FULL, SEED_BATCH, SEED_MIXED = 1001, 1002, 1003, 1004
+
+
+@dataclasses.dataclass
+class ColumnCase:
+    net: Network
+    col: Column
+    y0: np.ndarray
+    cfg: Config
+
+
+def small_network() -> Network:
+    return synthetic_network(30, 150, 0, 0, seed=SEED_SMALL)
+
+
+def full_network() -> Network:
+    return synthetic_network(93, 510, 90, 20, seed=SEED_FULL)
+
+
+def mixed_networks() -> List[Network]:
+    return [synthetic_network(40, 200, 0, 0, seed=SEED_MIXED),
+            full_network(),
+            synthetic_network(150, 935, 165, 0, seed=SEED_MIXED + 2, extra_elements=3)]
+
+
+def _case(net, nz, T_bot, T_top, Kzz_bot, Kzz_top, rng, cfg, tau=False, T_profile=None):
+    ymix0 = initial_mixing(net.ni, nz, rng)
+    col = build_column(net, nz, T_bot, T_top, P_b=1e9, P_t=1e4, Kzz_bot=Kzz_bot,
+                       Kzz_top=Kzz_top, ymix0=ymix0, cfg=cfg,
+                       tau_top=1e-4 if tau else None, tau_bot=1e2 if tau else None,
+                       T_profile=T_profile)
+    y0 = col.M[:, None] * ymix0
+    return ColumnCase(net=net, col=col, y0=y0, cfg=cfg)
+
+
+def small_case(nz: int = 60, use_moldiff: bool = True, net: Optional[Network] = None) -> ColumnCase:
+    cfg = Config(use_moldiff=use_moldiff)
+    rng = np.random.default_rng(np.random.SeedSequence(SEED_SMALL).spawn(1)[0])
+    return _case(net or small_network(), nz, 2000., 500., 1e7, None, rng, cfg)
+
+
+def full_case(nz: int = 150, net: Optional[Network] = None) -> ColumnCase:
+    cfg = Config()
+    rng = np.random.default_rng(np.random.SeedSequence(SEED_FULL).spawn(1)[0])
+    return _case(net or full_network(), nz, 2000., 500., 1e6, 1e10, rng, cfg, tau=True)
+
+
+def iso_gradient_profile(nz: int, T_top: float, T_bot: float, P_b=1e9, P_t=1e4, P_iso=1e5):
+    """T isothermal at T_top above P_iso, log-linear in P down to T_bot at P_b."""
+    pco = np.logspace(np.log10(P_b), np.log10(P_t), nz)
+    f = np.clip((np.log10(pco) - np.log10(P_iso)) / (np.log10(P_b) - np.log10(P_iso)), 0.0, 1.0)
+    return T_top + (T_bot - T_top) * f
+
+
+def batched_cases(ncols: int = 256, net: Optional[Network] = None, nz_range=(80, 200),
+                  seed: int = SEED_BATCH) -> List[ColumnCase]:
+    net = net or full_network()
+    cfg = Config()
+    out = []
+    for s in np.random.SeedSequence(seed).spawn(ncols):
+        rng = np.random.default_rng(s)
+        nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
+        T_top, T_bot = rng.uniform(400., 2500., 2)
+        K = 10.0 ** rng.uniform(5, 11)
+        prof = iso_gradient_profile(nz, T_top, T_bot)
+        out.append(_case(net, nz, T_bot, T_top, K, None, rng, cfg, tau=True, T_profile=prof))
+    return out
+
+
+def mixed_cases(ncols: int = 512, nets: Optional[List[Network]] = None,
+                seed: int = SEED_MIXED) -> List[ColumnCase]:
+    nets = nets or mixed_networks()
+    cfg = Config()
+    out = []
+    for c, s in enumerate(np.random.SeedSequence(seed).spawn(ncols)):
+        rng = np.random.default_rng(s)
+        net = nets[c % len(nets)]
+        nz = int(rng.integers(100, 151))
+        T_top, T_bot = rng.uniform(400., 2500., 2)
+        K = 10.0 ** rng.uniform(5, 11)
+        prof = iso_gradient_profile(nz, T_top, T_bot)
+        out.append(_case(net, nz, T_bot, T_top, K, None, rng, cfg, tau=False, T_profile=prof))
+    return out
