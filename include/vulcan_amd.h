@@ -1,0 +1,238 @@
+/*
+ * vulcan_amd: MI355X (gfx950) kernels for VULCAN's Ros2 kinetics-plus-
+ * diffusion step.  Plain C ABI for ctypes / any FFI.
+ *
+ * Conventions
+ *  - Every array argument named d_* is a device pointer owned by the caller
+ *    (for example a PyTorch-ROCm tensor's data_ptr()).  The library owns only
+ *    the immutable network tables (vk_plan), the column layout (vk_columns)
+ *    and the integrator workspace (vk_integrator).
+ *  - Layers of all columns are stacked: column c owns layers
+ *    [off[c], off[c] + nz[c]) of every per-layer array.  Per-layer arrays
+ *    are [L] or [L][ni] (species fastest), rate arrays are [L][nr] with
+ *    forward k at 2p and reverse k at 2p+1 for reaction pair p (VULCAN's
+ *    k[2p+1], k[2p+2]).  Interface quantities of a column (Kzz, Dzz, dzi, Ti,
+ *    Hpi: nz-1 values) sit in the first nz-1 layer slots of that column.
+ *  - Jacobian / lhs blocks are [L][ni][ni] row-major: entry (i, m) is
+ *    d f_i / d y_m of layer l; Lsub[l] (Usup[l]) is the diagonal of the
+ *    coupling block to layer l-1 (l+1).
+ *  - Kernels are asynchronous on the given stream (hipStream_t passed as
+ *    void*; NULL = default stream).  Return value: 0 = ok, < 0 = negated HIP
+ *    error or VK_E* code below.  Numerical failure is never an error: it is
+ *    reported through the state (negative y, delta, element loss), as in the
+ *    reference.
+ *  - A plan, columns object or integrator may be used by one host thread at a
+ *    time.
+ */
+#ifndef VULCAN_AMD_H
+#define VULCAN_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VK_EINVAL   (-1001)   /* bad argument / shape            */
+#define VK_ENOMEM   (-1002)   /* device allocation failed        */
+#define VK_ETOOBIG  (-1003)   /* ni beyond the compiled maximum  */
+
+/* rate-expression kinds (par[p][0..7] = A n E Ainf ninf Einf Fc J0) */
+#define VK_KIND_ARRH   0  /* op.py:166               */
+#define VK_KIND_LIND   1  /* op.py:176-182           */
+#define VK_KIND_K0     2  /* op.py:185-186           */
+#define VK_KIND_TROE   3  /* op.py:203-207 form      */
+#define VK_KIND_PHOTO  4  /* J0 exp(-tau), op.py:2786 */
+#define VK_KIND_CH3OH  5  /* op.py:197-207           */
+
+typedef struct vk_plan vk_plan;
+typedef struct vk_columns vk_columns;
+typedef struct vk_integrator vk_integrator;
+
+/* Host description of a network (copied into device tables by
+ * vk_plan_create).  Terms: species index, -1 = M, -2 = empty slot. */
+typedef struct {
+    int ni;                 /* species                                   */
+    int npairs;             /* reversible pairs (nr = 2*npairs rate rows) */
+    int natom;              /* monitored elements                        */
+    const int *reac;        /* [npairs][4] terms as written              */
+    const int *prod;        /* [npairs][4]                               */
+    const int *kind;        /* [npairs]                                  */
+    const double *par;      /* [npairs][8]                               */
+    const int *rev;         /* [npairs] 1: k_r = k_f / K_eq, 0: k_r = 0  */
+    const double *nasa9;    /* [ni][20] low [0:10], high [10:20]         */
+    const double *atom_mat; /* [ni][natom] element counts                */
+    const double *ms;       /* [ni] molar mass (g/mol)                   */
+    const double *alpha;    /* [ni] thermal diffusion factor             */
+    /* derived tables (see vulcan_amd/network.py: build_tables) */
+    const int *chem_ptr;    /* [ni+1]   */
+    const int *chem_pair;   /* [nchem]  */
+    const int *chem_sign;   /* [nchem]  */
+    int nchem;
+    const int *slot_pair;   /* [nslot]  */
+    const int *slot_sp;     /* [nslot]  */
+    int nslot;
+    const int *jac_map;     /* [ni*ni]  */
+    const int *jac_ptr;     /* [njnz+1] */
+    const int *jac_slot;    /* [njac]   */
+    const double *jac_coef; /* [njac]   */
+    int njnz;
+    int njac;
+} vk_network_desc;
+
+/* Device atmosphere arrays of a set of columns (all [L] unless noted). */
+typedef struct {
+    const double *Tco;
+    const double *M;        /* third body = total number density n_0   */
+    const double *Kzz;      /* interfaces                              */
+    const double *Dzz;      /* [L][ni] interfaces                      */
+    double *dzi;            /* interfaces (updated by update_mu_dz)    */
+    const double *Ti;       /* interfaces                              */
+    double *Hpi;            /* interfaces (updated)                    */
+    double *g;              /* layers (updated)                        */
+    double *Hp;             /* layers (updated)                        */
+    const double *tau;      /* photolysis optical depth, may be NULL   */
+} vk_atm;
+
+/* Column geometry inputs for update_mu_dz (per column / per layer). */
+typedef struct {
+    const double *lnp;      /* [L] log(pico[l]/pico[l+1])              */
+    double *zco;            /* [L + ncols] interface heights (nz+1 per column, offset off[c]+c) */
+    double *dz;             /* [L]                                     */
+    double *mu;             /* [L]                                     */
+    const double *gs;       /* [ncols]                                 */
+    const double *Rp;       /* [ncols]                                 */
+    const int *pref;        /* [ncols]                                 */
+} vk_geom;
+
+/* Numerical knobs (vulcan_cfg.py:124-159; names as there). */
+typedef struct {
+    double dt_min, dt_max, dt_var_min, dt_var_max;
+    double rtol, atol, mtol, mtol_conv, pos_cut, nega_cut, loss_eps;
+    double trun_min, runtime, st_factor, yconv_cri, slope_cri, yconv_min, flux_cri;
+    int count_min, count_max, conv_step, update_frq;
+    int use_moldiff;
+    int use_conv;           /* evaluate the steady-state test (needs history)  */
+    int use_update_mu;      /* run update_mu_dz every update_frq steps         */
+    int refine;             /* refinement steps per linear solve (1 = default) */
+} vk_params;
+
+/* Per-column integrator state (device resident; mirrors var/para fields). */
+typedef struct {
+    double t, dt, delta, longdy, longdydt, small_y, nega_y, pad0;
+    int count, attempts, nega_count, loss_count, delta_count, end_case;
+    int retry;      /* 1: the next attempt retries the current step       */
+    int done;       /* 1: stop() returned true, column no longer steps     */
+} vk_colstate;
+
+/* ---- plan / layout -------------------------------------------------------*/
+int vk_plan_create(const vk_network_desc *net, vk_plan **out);   /* make_chem_funs.py output + op.ReadRate tables */
+int vk_plan_destroy(vk_plan *plan);
+int vk_plan_info(const vk_plan *plan, int *ni, int *nr, int *max_ni);
+int vk_columns_create(int ncols, const int *nz, vk_columns **out);
+int vk_columns_destroy(vk_columns *cols);
+int vk_columns_layers(const vk_columns *cols, int *nlayers);
+
+/* ---- reference functions, one kernel each ------------------------------- */
+/* op.ReadRate.read_rate + rev_rate (op.py:63-309), photolysis k = J (op.py:2786) */
+int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+             double *d_k, void *stream);
+/* chem_funs.chemdf(y, M, k) (chem_funs.py:931-1000) */
+int vk_chemdf(const vk_plan *plan, const vk_columns *cols, const double *d_y,
+              const double *d_M, const double *d_k, double *d_dydt, void *stream);
+/* ODESolver.diffdf / diffdf_no_mol (op.py:1496-1597 / 1438-1494) */
+int vk_diffdf(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+              const double *d_y, int use_moldiff, double *d_out, void *stream);
+/* chemdf + diffdf fused; with d_k1 != NULL also - 2/(r dt_c) k1 (op.py:2892, 2918-2928) */
+int vk_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+           const double *d_y, const double *d_k, int use_moldiff,
+           const double *d_k1, const double *d_dt, double *d_out, void *stream);
+/* chem_funs.symjac diagonal blocks (chem_funs.py:3222-7453); neg_symjac = -J */
+int vk_jac(const vk_plan *plan, const vk_columns *cols, const double *d_y,
+           const double *d_M, const double *d_k, double *d_J, void *stream);
+/* ODESolver.lhs_jac_tot / lhs_jac_no_mol (op.py:1973-2042 / 2122-2166) as blocks */
+int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+           const double *d_y, const double *d_k, int use_moldiff,
+           const double *d_dt, double *d_D, double *d_Lsub, double *d_Usup, void *stream);
+/* replaces store_bandM + LAPACK gbtrf inside scipy.linalg.solve_banded
+ * (op.py:2833-2858, 2914): twisted block-Thomas factorisation (a lower
+ * chain from layer 0 and an upper chain from layer nz-1 meet at layer
+ * nz/2); d_Dinv[l] receives the inverse of layer l's Schur complement,
+ * stored transposed (d_Dinv[l][m][i] = S_l^-1[i][m]); d_D is not modified;
+ * d_info[c] = 1 + layer of a singular block, or 0 */
+int vk_block_factor(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                    const double *d_Lsub, const double *d_Usup, double *d_Dinv,
+                    int *d_info, void *stream);
+/* replaces the gbtrs half of solve_banded (op.py:2914, 2929): rhs <- lhs^-1 rhs */
+int vk_block_solve(const vk_plan *plan, const vk_columns *cols, const double *d_Dinv,
+                   const double *d_Lsub, const double *d_Usup, double *d_rhs, void *stream);
+/* r = b - lhs x accumulated in double-double: the residual of one step of
+ * iterative refinement, which brings the block solve to the accuracy of
+ * LAPACK's banded LU or better on ill-conditioned blocks (no counterpart in
+ * the reference; used inside the Ros2 attempt for both solves) */
+int vk_block_residual(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                      const double *d_Lsub, const double *d_Usup, const double *d_x,
+                      const double *d_b, double *d_r, void *stream);
+
+/* ---- integrator (Ros2.solver / one_step / step_size, Integration loop) --- */
+int vk_integrator_create(const vk_plan *plan, const vk_columns *cols, int history,
+                         vk_integrator **out);
+int vk_integrator_destroy(vk_integrator *it);
+/* Ros2.solver (op.py:2860-3007): one attempt for every column that is not
+ * done; y is replaced by sol, ymix by sol/sum(sol), st.delta set.  The
+ * attempt does not clip or accept. */
+int vk_ros2_solver(vk_integrator *it, const vk_atm *atm, const vk_params *prm,
+                   const double *d_k, double *d_y, double *d_ymix,
+                   vk_colstate *d_st, void *stream);
+/* Loop body of Integration.__call__ (op.py:813-923) including Ros2.one_step
+ * (op.py:3091-3103), clip/loss/step_ok/step_reject (op.py:2447-2534),
+ * update_mu_dz (op.py:944-984), renormalisation (909-914), save_step,
+ * step_size (3105-3125) and stop/conv (1018-1085).  Runs n_attempts
+ * solver attempts (accepted or not) on every column that is not done.
+ * d_y_prev and d_atom_* are per-column state arrays ([L][ni],
+ * [ncols][natom]). */
+int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom,
+                 const vk_params *prm, const double *d_k, double *d_y,
+                 double *d_ymix, double *d_y_prev, double *d_atom_ini,
+                 double *d_atom_loss, double *d_atom_loss_prev,
+                 vk_colstate *d_st, int n_attempts, void *stream);
+/* update_mu_dz (op.py:944-984) for every column that is not done */
+int vk_update_mu_dz(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+                    const vk_geom *geom, const double *d_ymix, const vk_colstate *d_st,
+                    int force, void *stream);
+/* element sums: atom_sum[c][a] = sum_{l,i} atom_mat[i][a] y[l][i] (op.py:2472-2487, build_atm.py:283-291) */
+int vk_atom_sums(const vk_plan *plan, const vk_columns *cols, const double *d_y,
+                 double *d_atom_sum, void *stream);
+/* With timing on, vk_integrate records HIP events around the factor kernel
+ * of its last attempt (on the call's stream) and, before returning, waits
+ * for them and adds the duration to the totals that vk_integrator_stats
+ * reports (summed milliseconds, number of timed launches). */
+int vk_integrator_set_timing(vk_integrator *it, int on);
+int vk_integrator_stats(const vk_integrator *it, double *factor_ms, long long *factor_launches);
+
+/* ---- warm-start MLP (no counterpart in the reference: it seeds the plain
+ * Ros2 integration, which stays the fallback and the judge) --------------
+ * BF16 tensors are passed as raw 16-bit arrays.  GEMM: C[m][n] = sum_k
+ * A(m,k) B(k,n) (+ bias[n]) (+ C if accumulate) with A(m,k) at
+ * A[m*sam + k*sak] and B(k,n) at B[k*sbk + n*sbn]; FP32 accumulation on
+ * BF16 MFMA; with d_Yact != NULL also Yact = SiLU(C) as BF16. */
+int vk_mlp_gemm(int M, int N, int K, const void *d_A, long sam, long sak,
+                const void *d_B, long sbk, long sbn, float *d_C, long ldc,
+                const float *d_bias, int accumulate, void *d_Yact, long ldy, void *stream);
+/* dZ = dY * SiLU'(Z) (FP32 and BF16 copies) */
+int vk_mlp_silu_bwd(long n, const float *d_Z, const float *d_dY, float *d_dZ, void *d_dZb,
+                    void *stream);
+/* masked MSE: dY = 2 w (Y - T) inv_count, loss partial sums per 256 entries */
+int vk_mlp_mse(long n, const float *d_Y, const float *d_T, const float *d_W, float inv_count,
+               float *d_dY, void *d_dYb, float *d_loss_part, void *stream);
+/* out[n] = sum_m X[m][n] (bias gradients) */
+int vk_mlp_colsum(int M, int N, const float *d_X, float *d_out, void *stream);
+/* FP32 -> BF16 (round to nearest even) */
+int vk_mlp_to_bf16(long n, const float *d_x, void *d_y, void *stream);
+/* one Adam step (bias-corrected, decoupled weight decay wd) on FP32 master
+ * weights; refreshes the BF16 copy d_wb when not NULL */
+int vk_mlp_adam(long n, float *d_w, const float *d_g, float *d_m, float *d_v, float lr, float b1,
+                float b2, float eps, int step, float wd, void *d_wb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

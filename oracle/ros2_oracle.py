@@ -101,7 +101,7 @@ def forward_rate(net: nw.Network, p: int, T, M, tau=None):
         ff = np.exp(np.log(Fc) / (1. + (np.log(k0 * M / kinf) / nn) ** 2))   # op.py:205
         return k0 / (1 + k0 * M / kinf) * ff                                 # op.py:207
     if kind == nw.KIND_PHOTO:
-        return par[nw.P_J0] * np.exp(-tau)
+        return par[nw.P_J0] * np.exp(-(0.0 if tau is None else tau)) * np.ones_like(T)
     if kind == nw.KIND_CH3OH:                                                # op.py:197-207
         k0 = 1.932E3 * T ** -9.88 * np.exp(-7544. / T) + 5.109E-11 * T ** -6.25 * np.exp(-1433. / T)
         kinf = 1.031E-10 * T ** -0.018 * np.exp(16.74 / T)
@@ -441,10 +441,19 @@ def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False):
     J = jac_blocks(net, y, M, k, tables)
     D, L, U = lhs_blocks(col, y, h, J, cfg.use_moldiff)
     if dense_lhs:
-        # the reference's cost profile: dense lhs, then the column-loop repack
+        # the reference's cost profile: a dense (ni*nz)^2 neg_symjac, then the
+        # diagonal/diffusion updates and the column-loop band repack; the
+        # values written are the blocks above, so both paths give equal bits
         lhs = neg_symjac_dense(J)
-        Dd, Ld, Ud = lhs_blocks(col, y, h, np.zeros_like(J), cfg.use_moldiff)
-        lhs += lhs_dense(Dd, Ld, Ud)
+        nz_, ni_ = y.shape
+        rows = np.arange(ni_)
+        for j in range(nz_):
+            s = slice(j * ni_, (j + 1) * ni_)
+            lhs[s, s] = D[j]
+            if j > 0:
+                lhs[j * ni_ + rows, (j - 1) * ni_ + rows] = L[j]
+            if j < nz_ - 1:
+                lhs[j * ni_ + rows, (j + 1) * ni_ + rows] = U[j]
         lhs_b, bw = store_bandM(lhs, net.ni, y.shape[0])
         del lhs
     else:

@@ -394,3 +394,17 @@ def test_reject_branches_and_give_up():
     st.dt = 1.5e-14
     assert O.step_reject(st, cfg) is True and st.loss_count == 1
     assert st.dt == cfg.dt_min and np.all(st.y >= 0)
+
+
+def test_dense_reference_cost_path_equals_band_path():
+    """The CPU baseline runs the reference's cost profile (dense (ni*nz)^2 lhs,
+    column-loop band repack); it must give the same bits as the band path."""
+    c = W.small_case(nz=10)
+    cfg = c.cfg
+    k = O.rates(c.net, c.col.Tco, c.col.M, c.col.tau)
+    out = []
+    for dense in (False, True):
+        st = O.new_state(c.net, c.y0, k, cfg)
+        O.ros2_solver(c.net, c.col, st, cfg, dense_lhs=dense)
+        out.append((st.y.copy(), st.delta))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]

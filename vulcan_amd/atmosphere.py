@@ -89,6 +89,11 @@ def mol_diff_H2(Tco, n_0, species, mass, use_moldiff=True):
     return Dzz, alpha, ms
 
 
+def thermal_alpha(species, mass) -> np.ndarray:
+    """Thermal diffusion factors for an H2 background (build_atm.py:685-690)."""
+    return mol_diff_H2(np.array([1000., 1000.]), np.array([1., 1.]), list(species), mass, True)[1]
+
+
 def mu_g_dz(ymix, ms, Tco, pico, gs, Rp, pref_indx):
     """Mean mass, gravity, scale height and layer geometry (build_atm.py:515-615)."""
     nz = Tco.shape[0]

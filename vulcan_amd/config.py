@@ -57,6 +57,9 @@ class Config:
     flux_cri: float = 0.1
     conver_ignore: List[str] = dataclasses.field(default_factory=list)
     rtol: float = 0.25                # vulcan_cfg.py:156
+    # iterative-refinement steps per block solve (double-double residual);
+    # no counterpart in the reference, whose banded LU needs none
+    refine: int = 1
 
     def check_supported(self) -> None:
         """Refuse switches whose physics is not on the accelerated path."""

@@ -1,0 +1,386 @@
+// Rate constants, chemistry + diffusion right-hand side, Jacobian and Ros2
+// left-hand-side assembly.  One workgroup per layer for the per-layer
+// kernels: the layer's y row and its reaction rates live in LDS, species
+// rows are gathered from per-species CSR lists (no atomics, fixed order).
+#include "vk_internal.h"
+#include "vk_device.h"
+
+using namespace vk;
+
+namespace {
+
+constexpr int BLK = 256;
+
+// ---------------------------------------------------------------------------
+// rates (op.py:145-207, 287-309; photolysis op.py:2786)
+// ---------------------------------------------------------------------------
+
+__global__ void k_grt(int L, int ni, const double *__restrict__ Tco, const double *__restrict__ nasa9,
+                      double *__restrict__ grt) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= L * ni) return;
+    int l = idx / ni, s = idx - l * ni;
+    grt[idx] = g_RT(Tco[l], nasa9 + 20 * s);
+}
+
+__device__ inline double arrh(double A, double n, double E, double T) {
+    return A * pow(T, n) * exp(-E / T);
+}
+
+__global__ void k_rates(int L, int npairs, int ni, const int *__restrict__ kind, const double *__restrict__ par,
+                        const int *__restrict__ rev, const int *__restrict__ reac, const int *__restrict__ prod,
+                        const double *__restrict__ Tco, const double *__restrict__ M, const double *__restrict__ tau,
+                        const double *__restrict__ grt, double *__restrict__ k) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= L * npairs) return;
+    int l = idx / npairs, p = idx - l * npairs;
+    double T = Tco[l], m = M[l];
+    const double *pa = par + 8 * p;
+    double kf;
+    switch (kind[p]) {
+    case VK_KIND_ARRH:
+    case VK_KIND_K0:
+        kf = arrh(pa[0], pa[1], pa[2], T);
+        break;
+    case VK_KIND_LIND: {
+        double k0 = arrh(pa[0], pa[1], pa[2], T);
+        double kinf = arrh(pa[3], pa[4], pa[5], T);
+        kf = k0 / (1 + k0 * m / kinf);
+        break;
+    }
+    case VK_KIND_TROE: {
+        double k0 = arrh(pa[0], pa[1], pa[2], T);
+        double kinf = arrh(pa[3], pa[4], pa[5], T);
+        double Fc = pa[6];
+        double nn = 0.75 - 1.27 * log(Fc);
+        double q = log(k0 * m / kinf) / nn;
+        double ff = exp(log(Fc) / (1. + q * q));
+        kf = k0 / (1 + k0 * m / kinf) * ff;
+        break;
+    }
+    case VK_KIND_PHOTO:
+        kf = pa[7] * exp(-(tau ? tau[l] : 0.0));
+        break;
+    case VK_KIND_CH3OH: {
+        double k0 = 1.932E3 * pow(T, -9.88) * exp(-7544. / T) + 5.109E-11 * pow(T, -6.25) * exp(-1433. / T);
+        double kinf = 1.031E-10 * pow(T, -0.018) * exp(16.74 / T);
+        double Fc = 0.1855 * exp(-T / 155.8) + 0.8145 * exp(-T / 1675.) + exp(-4531. / T);
+        double nn = 0.75 - 1.27 * log(Fc);
+        double q = log(k0 * m / kinf) / nn;
+        double ff = exp(log(Fc) / (1. + q * q));
+        kf = k0 / (1 + k0 * m / kinf) * ff;
+        break;
+    }
+    default:
+        kf = 0.0;
+    }
+    double kr = 0.0;
+    if (rev[p]) {
+        // K = exp(-(-g_r1 - g_r2 + g_p1 + g_p2)) * (corr T)^(n_r - n_p)  (make_chem_funs.py:568-580)
+        const double *g = grt + (size_t)l * ni;
+        double acc = 0.0;
+        bool first = true;
+        int nre = 0, npr = 0;
+        for (int q = 0; q < 4; q++) {
+            int t = reac[4 * p + q];
+            if (t >= 0) { acc = first ? -1 * g[t] : acc - 1 * g[t]; first = false; nre++; }
+        }
+        for (int q = 0; q < 4; q++) {
+            int t = prod[4 * p + q];
+            if (t >= 0) { acc = first ? 1 * g[t] : acc + 1 * g[t]; first = false; npr++; }
+        }
+        double K = exp(-(acc));
+        int dn = nre - npr;
+        if (dn != 0) {
+            double x = CORR * T;
+            double f;
+            if (dn == 1) f = x;
+            else if (dn == -1) f = 1.0 / x;
+            else if (dn == 2) f = x * x;
+            else f = pow(x, (double)dn);
+            K = K * f;
+        }
+        kr = kf / K;
+    }
+    k[(size_t)l * 2 * npairs + 2 * p] = kf;
+    k[(size_t)l * 2 * npairs + 2 * p + 1] = kr;
+}
+
+// ---------------------------------------------------------------------------
+// per-layer helpers
+// ---------------------------------------------------------------------------
+
+struct LayerCtx {
+    int l, c, off, nz, j;
+};
+
+__device__ inline LayerCtx layer_ctx(const int *off, const int *layer_col, int l) {
+    LayerCtx x;
+    x.l = l;
+    x.c = layer_col[l];
+    x.off = off[x.c];
+    x.nz = off[x.c + 1] - x.off;
+    x.j = l - x.off;
+    return x;
+}
+
+// v_p = k_f * prod(reac) - k_r * prod(prod), left to right (make_chem_funs.py:293-315)
+__device__ inline double pair_rate(const int *reac, const int *prod, const double *kl, const double *yl,
+                                   double m, int p) {
+    double a = kl[2 * p];
+    for (int q = 0; q < 4; q++) {
+        int t = reac[4 * p + q];
+        if (t == -2) continue;
+        a = a * (t == -1 ? m : yl[t]);
+    }
+    double b = kl[2 * p + 1];
+    for (int q = 0; q < 4; q++) {
+        int t = prod[4 * p + q];
+        if (t == -2) continue;
+        b = b * (t == -1 ? m : yl[t]);
+    }
+    return a - b;
+}
+
+// d/dy_m of kcol * prod(terms)
+__device__ inline double dside(const int *terms, double kcol, const double *yl, double m, int sp, bool &any) {
+    double tot = 0.0;
+    any = false;
+    for (int q = 0; q < 4; q++) {
+        if (terms[q] != sp) continue;
+        double acc = kcol;
+        for (int q2 = 0; q2 < 4; q2++) {
+            int t2 = terms[q2];
+            if (q2 == q || t2 == -2) continue;
+            acc = acc * (t2 == -1 ? m : yl[t2]);
+        }
+        tot = any ? tot + acc : acc;
+        any = true;
+    }
+    return tot;
+}
+
+// ---------------------------------------------------------------------------
+// chemdf + diffdf (+ stage-2 term), one workgroup per layer
+// mode bit 0: include chemistry, bit 1: include diffusion
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK) k_rhs(
+    int ni, int npairs, const int *__restrict__ off, const int *__restrict__ layer_col,
+    const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ chem_ptr,
+    const int *__restrict__ chem_pair, const int *__restrict__ chem_sign, const double *__restrict__ ms,
+    const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
+    const double *__restrict__ M, int mode, int mol, const double *__restrict__ k1,
+    const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ out) {
+    extern __shared__ double smem[];
+    double *yl = smem;              // ni
+    double *v = smem + VK_NI_MAX;   // npairs
+    __shared__ double ysum3[3];
+    const int l = blockIdx.x;
+    LayerCtx x = layer_ctx(off, layer_col, l);
+    if (st && st[x.c].done) return;
+    const double *yrow = y + (size_t)l * ni;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
+    if (mode & 2) {
+        if (threadIdx.x < 3) {
+            int jj = x.j - 1 + threadIdx.x;
+            double s = 0.0;
+            if (jj >= 0 && jj < x.nz) s = pw_sum(y + (size_t)(x.off + jj) * ni, ni);
+            ysum3[threadIdx.x] = s;
+        }
+    }
+    __syncthreads();
+    if (mode & 1) {
+        const double *kl = k + (size_t)l * 2 * npairs;
+        double m = M[l];
+        for (int p = threadIdx.x; p < npairs; p += blockDim.x) v[p] = pair_rate(reac, prod, kl, yl, m, p);
+        __syncthreads();
+    }
+    ColAtm ca = col_atm(atm, x.off, x.nz, ni);
+    double c2 = 0.0;
+    if (k1) c2 = 2. / (R_ROS * dtc[x.c]);
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        double chem = 0.0;
+        if (mode & 1) {
+            int lo = chem_ptr[i], hi = chem_ptr[i + 1];
+            if (lo < hi) {
+                chem = chem_sign[lo] * v[chem_pair[lo]];
+                for (int e = lo + 1; e < hi; e++) chem = chem + chem_sign[e] * v[chem_pair[e]];
+            }
+        }
+        double val = chem;
+        if (mode & 2) {
+            double ym = x.j > 0 ? y[(size_t)(l - 1) * ni + i] : 0.0;
+            double yp = x.j < x.nz - 1 ? y[(size_t)(l + 1) * ni + i] : 0.0;
+            double d = diff_value(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ym, yl[i], yp, ms[i], alpha[i], mol != 0);
+            val = (mode & 1) ? chem + d : d;
+        }
+        if (k1) val = val - c2 * k1[(size_t)l * ni + i];
+        out[(size_t)l * ni + i] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian blocks (mode 0: J = d chemdf/dy) or Ros2 lhs blocks (mode 1:
+// D = c0 I - J - J_diff, Lsub, Usup), one workgroup per layer
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK) k_jac_lhs(
+    int ni, int npairs, int nslot, const int *__restrict__ off, const int *__restrict__ layer_col,
+    const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
+    const int *__restrict__ slot_sp, const int *__restrict__ jac_map, const int *__restrict__ jac_ptr,
+    const int *__restrict__ jac_slot, const double *__restrict__ jac_coef, const double *__restrict__ ms,
+    const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
+    const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
+    const vk_colstate *__restrict__ st, double *__restrict__ D, double *__restrict__ Lsub,
+    double *__restrict__ Usup) {
+    extern __shared__ double smem[];
+    double *yl = smem;               // ni
+    double *dv = smem + VK_NI_MAX;   // nslot
+    __shared__ double ysum3[3];
+    const int l = blockIdx.x;
+    LayerCtx x = layer_ctx(off, layer_col, l);
+    if (st && st[x.c].done) return;
+    const double *yrow = y + (size_t)l * ni;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
+    if (mode == 1 && threadIdx.x < 3) {
+        int jj = x.j - 1 + threadIdx.x;
+        double s = 0.0;
+        if (jj >= 0 && jj < x.nz) s = pw_sum(y + (size_t)(x.off + jj) * ni, ni);
+        ysum3[threadIdx.x] = s;
+    }
+    __syncthreads();
+    const double *kl = k + (size_t)l * 2 * npairs;
+    const double m = M[l];
+    for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
+        int p = slot_pair[s], sp = slot_sp[s];
+        bool a1, a2;
+        double dr = dside(reac + 4 * p, kl[2 * p], yl, m, sp, a1);
+        double dp = dside(prod + 4 * p, kl[2 * p + 1], yl, m, sp, a2);
+        dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
+    }
+    __syncthreads();
+    const int nn = ni * ni;
+    double *Dl = D + (size_t)l * nn;
+    double c0 = 0.0;
+    if (mode == 1) c0 = 1. / (R_ROS * dtc[x.c]);
+    ColAtm ca = col_atm(atm, x.off, x.nz, ni);
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        int jm = jac_map[e];
+        double J = 0.0;
+        if (jm >= 0) {
+            int lo = jac_ptr[jm], hi = jac_ptr[jm + 1];
+            J = jac_coef[lo] * dv[jac_slot[lo]];
+            for (int q = lo + 1; q < hi; q++) J = J + jac_coef[q] * dv[jac_slot[q]];
+        }
+        if (mode == 0) {
+            Dl[e] = J;
+            continue;
+        }
+        int i = e / ni, mm = e - i * ni;
+        double val = -J;
+        if (i == mm) {
+            val = c0 + val;
+            LhsDiff r = lhs_diff(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ms[i], alpha[i], mol != 0);
+            val = val - r.d0;
+            if (r.has_mol) val = val - r.d1;
+        }
+        Dl[e] = val;
+    }
+    if (mode == 1) {
+        for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+            LhsDiff r = lhs_diff(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ms[i], alpha[i], mol != 0);
+            double u = 0.0, lo = 0.0;
+            if (r.has_u) { u = 0.0 - r.u0; if (r.has_mol) u = u - r.u1; }
+            if (r.has_l) { lo = 0.0 - r.l0; if (r.has_mol) lo = lo - r.l1; }
+            Usup[(size_t)l * ni + i] = u;
+            Lsub[(size_t)l * ni + i] = lo;
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host entry points
+// ---------------------------------------------------------------------------
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+static size_t rhs_smem(const vk_plan *p) { return sizeof(double) * (VK_NI_MAX + (size_t)p->npairs); }
+static size_t jac_smem(const vk_plan *p) { return sizeof(double) * (VK_NI_MAX + (size_t)p->nslot); }
+
+extern "C" int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, double *d_k, void *stream) {
+    if (!plan || !cols || !atm || !d_k) return VK_EINVAL;
+    int L = cols->L;
+    double *grt = nullptr;
+    VK_CHECK(hipMallocAsync((void **)&grt, sizeof(double) * (size_t)L * plan->ni, S(stream)));
+    int n1 = L * plan->ni;
+    hipLaunchKernelGGL(k_grt, dim3((n1 + 255) / 256), dim3(256), 0, S(stream), L, plan->ni, atm->Tco, plan->nasa9, grt);
+    int n2 = L * plan->npairs;
+    hipLaunchKernelGGL(k_rates, dim3((n2 + 255) / 256), dim3(256), 0, S(stream), L, plan->npairs, plan->ni,
+                       plan->kind, plan->par, plan->rev, plan->reac, plan->prod, atm->Tco, atm->M, atm->tau, grt, d_k);
+    VK_CHECK(hipGetLastError());
+    VK_CHECK(hipFreeAsync(grt, S(stream)));
+    return 0;
+}
+
+int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
+                  const double *k, int mode, int mol, const double *k1, const double *dtc,
+                  const vk_colstate *st, double *out, hipStream_t s) {
+    vk_atm a = *atm;
+    hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, cols->d_off,
+                       cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_pair, plan->chem_sign,
+                       plan->ms, plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, st, out);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
+                      const double *k, const double *M, int mode, int mol, const double *dtc,
+                      const vk_colstate *st, double *D, double *Lsub, double *Usup, hipStream_t s) {
+    vk_atm a{};
+    if (atm) a = *atm;
+    hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
+                       cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair, plan->slot_sp,
+                       plan->jac_map, plan->jac_ptr, plan->jac_slot, plan->jac_coef, plan->ms, plan->alpha, a, y, k,
+                       M, mode, mol, dtc, st, D, Lsub, Usup);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vk_chemdf(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
+                         const double *d_k, double *d_dydt, void *stream) {
+    if (!plan || !cols || !d_y || !d_M || !d_k || !d_dydt) return VK_EINVAL;
+    vk_atm a{};
+    a.M = d_M;
+    return vk_launch_rhs(plan, cols, &a, d_y, d_k, 1, 0, nullptr, nullptr, nullptr, d_dydt, S(stream));
+}
+
+extern "C" int vk_diffdf(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
+                         int use_moldiff, double *d_out, void *stream) {
+    if (!plan || !cols || !atm || !d_y || !d_out) return VK_EINVAL;
+    return vk_launch_rhs(plan, cols, atm, d_y, nullptr, 2, use_moldiff, nullptr, nullptr, nullptr, d_out, S(stream));
+}
+
+extern "C" int vk_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
+                      const double *d_k, int use_moldiff, const double *d_k1, const double *d_dt, double *d_out,
+                      void *stream) {
+    if (!plan || !cols || !atm || !d_y || !d_k || !d_out || (d_k1 && !d_dt)) return VK_EINVAL;
+    return vk_launch_rhs(plan, cols, atm, d_y, d_k, 3, use_moldiff, d_k1, d_dt, nullptr, d_out, S(stream));
+}
+
+extern "C" int vk_jac(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
+                      const double *d_k, double *d_J, void *stream) {
+    if (!plan || !cols || !d_y || !d_M || !d_k || !d_J) return VK_EINVAL;
+    return vk_launch_jac_lhs(plan, cols, nullptr, d_y, d_k, d_M, 0, 0, nullptr, nullptr, d_J, nullptr, nullptr,
+                             S(stream));
+}
+
+extern "C" int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
+                      const double *d_k, int use_moldiff, const double *d_dt, double *d_D, double *d_Lsub,
+                      double *d_Usup, void *stream) {
+    if (!plan || !cols || !atm || !d_y || !d_k || !d_dt || !d_D || !d_Lsub || !d_Usup) return VK_EINVAL;
+    return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, d_D, d_Lsub, d_Usup,
+                             S(stream));
+}

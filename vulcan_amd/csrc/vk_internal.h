@@ -1,0 +1,56 @@
+// Internal structures shared by the vulcan_amd HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "../../include/vulcan_amd.h"
+
+#define VK_NI_MAX 160        // largest network the kernels are compiled for
+#define VK_NATOM_MAX 8
+
+#define VK_CHECK(expr)                                                   \
+    do {                                                                 \
+        hipError_t e_ = (expr);                                          \
+        if (e_ != hipSuccess) return -(int)e_;                           \
+    } while (0)
+
+struct vk_plan {
+    int ni, npairs, nr, natom, nchem, nslot, njnz, njac;
+    int *reac, *prod, *kind, *rev;
+    double *par, *nasa9, *atom_mat, *ms, *alpha;
+    int *chem_ptr, *chem_pair, *chem_sign;
+    int *slot_pair, *slot_sp;
+    int *jac_map, *jac_ptr, *jac_slot;
+    double *jac_coef;
+    std::vector<void *> allocs;
+};
+
+struct vk_columns {
+    int ncols, L, max_nz;
+    std::vector<int> off;      // host copy, ncols+1
+    int *d_off;                // ncols+1
+    int *d_layer_col;          // L
+};
+
+// per-layer partial results of the finalize kernel
+enum { VKP_DELTA = 0, VKP_NEG = 1, VKP_SMALL = 2, VKP_NEGY = 3, VKP_ATOM = 4 };
+
+struct vk_integrator {
+    const vk_plan *plan;
+    const vk_columns *cols;
+    int history;               // ring length for conv (0 = no history)
+    int tcap;                  // capacity of t_time per column
+    double *D, *Dinv, *Lsub, *Usup;     // lhs blocks, inverse Schur complements, couplings
+    double *b1, *x1, *b2, *x2, *work;   // right-hand sides and solutions of the two stages
+    double *ztmp;                       // forward-sweep scratch of the twisted solve
+    double *yk2, *part;
+    int *info;
+    double *yhist;             // [history][L][ni]
+    double *thist;             // [ncols][tcap]
+    int *flags;                // [ncols] per-attempt decision
+    double *colscr;            // [ncols][16] per-column scratch
+    hipEvent_t ev0, ev1;
+    int timing;                // time the factor kernel of the last attempt per call
+    double factor_ms;          // summed duration of the timed factor launches
+    long long factor_launches; // number of timed factor launches
+    std::vector<void *> allocs;
+};

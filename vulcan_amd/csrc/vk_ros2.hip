@@ -1,0 +1,693 @@
+// Plan / column layout objects and the device-resident Ros2 state machine:
+// one attempt = rhs -> lhs -> factor -> solve -> rhs2 -> solve -> finalize
+// (op.py:2860-3007), then per column clip / loss / step_ok / step_reject
+// (op.py:2447-2534), update_mu_dz (944-984), renormalisation (909-914),
+// save_step (1087-1105), step_size (3105-3125) and stop/conv (1018-1085).
+#include <cstring>
+#include "vk_internal.h"
+#include "vk_device.h"
+
+using namespace vk;
+
+int vk_launch_rhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *, int, int,
+                  const double *, const double *, const vk_colstate *, double *, hipStream_t);
+int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *,
+                      const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
+                      hipStream_t);
+int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
+                     int *, const vk_colstate *, hipStream_t);
+int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
+                            const double *, const double *, double *, double *, double *, int, const vk_colstate *,
+                            hipStream_t);
+
+namespace {
+
+constexpr int NP = 4 + VK_NATOM_MAX;   // per-layer partials stride
+constexpr double C3 = 3. / (2. * R_ROS);
+constexpr double C1 = 1 / (2. * R_ROS);
+
+template <typename T>
+int dalloc(std::vector<void *> &v, T **p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+    if (e != hipSuccess) return VK_ENOMEM;
+    v.push_back(*p);
+    return 0;
+}
+
+template <typename T>
+int upload(std::vector<void *> &v, T **p, const T *src, size_t n) {
+    int r = dalloc(v, p, n);
+    if (r) return r;
+    if (n && src) {
+        hipError_t e = hipMemcpy(*p, src, sizeof(T) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return -(int)e;
+    }
+    return 0;
+}
+
+__device__ inline void layer_of(const int *off, const int *layer_col, int l, int &c, int &o, int &nz) {
+    c = layer_col[l];
+    o = off[c];
+    nz = off[c + 1] - o;
+}
+
+__global__ void k_gather_dt(int n, const vk_colstate *__restrict__ st, double *__restrict__ dt) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) dt[c] = st[c].dt;
+}
+
+// yk2 = y + k1/r  (op.py:2917)
+__global__ void k_yk2(int n, int ni, const int *__restrict__ layer_col, const double *__restrict__ y,
+                      const double *__restrict__ k1, const vk_colstate *__restrict__ st, double *__restrict__ yk2) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    if (st[layer_col[idx / ni]].done) return;
+    yk2[idx] = y[idx] + k1[idx] / R_ROS;
+}
+
+// sol, delta partial, ymix; with clip also the clip/loss partials (op.py:2932-2993, 2447-2487)
+__global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *__restrict__ off,
+                                                  const int *__restrict__ layer_col,
+                                                  const double *__restrict__ atom_mat, const double *y,
+                                                  const double *ymix_in, const double *__restrict__ k1,
+                                                  const double *__restrict__ k2, const double *__restrict__ yk2,
+                                                  vk_params prm, int clip, const vk_colstate *__restrict__ st,
+                                                  double *ynew, double *ymixnew, double *__restrict__ part) {
+    __shared__ double sol[VK_NI_MAX];
+    __shared__ double red[256];
+    __shared__ int negf[256];
+    __shared__ double sm[256], ng[256];
+    __shared__ double ysum;
+    const int l = blockIdx.x;
+    int c, o, nz;
+    layer_of(off, layer_col, l, c, o, nz);
+    if (st[c].done) return;
+    const size_t base = (size_t)l * ni;
+    double rmax = -1.0;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        double s = y[base + i] + C3 * k1[base + i] + C1 * k2[base + i];
+        double d = fabs(s - yk2[base + i]);
+        if (ymix_in[base + i] < prm.mtol) d = 0;
+        if (s < prm.atol) d = 0;
+        if (s > 0) rmax = fmax(rmax, d / s);
+        sol[i] = s;
+    }
+    red[threadIdx.x] = rmax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[(size_t)l * NP + VKP_DELTA] = red[0];
+        ysum = pw_sum(sol, ni);
+    }
+    __syncthreads();
+    const double ys = ysum;
+    if (!clip) {
+        for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+            ynew[base + i] = sol[i];
+            ymixnew[base + i] = sol[i] / ys;
+        }
+        return;
+    }
+    double small = 0.0, nega = 0.0;
+    int anyneg = 0;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        double v = sol[i];
+        double mx = v / ys;
+        if (v < prm.pos_cut && v >= 0) small += v;
+        if (v > prm.nega_cut && v <= 0) nega += v;
+        if (v < prm.pos_cut && v >= prm.nega_cut) v = 0.;
+        if (mx < prm.mtol && v < 0) v = 0.;
+        if (v < 0) anyneg = 1;
+        sol[i] = v;
+    }
+    sm[threadIdx.x] = small;
+    ng[threadIdx.x] = nega;
+    negf[threadIdx.x] = anyneg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s1 = 0.0, s2 = 0.0;
+        int nf = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) { s1 += sm[t]; s2 += ng[t]; nf |= negf[t]; }
+        part[(size_t)l * NP + VKP_NEG] = nf;
+        part[(size_t)l * NP + VKP_SMALL] = s1;
+        part[(size_t)l * NP + VKP_NEGY] = s2;
+        ysum = pw_sum(sol, ni);
+    }
+    if (threadIdx.x < natom) {
+        const int a = threadIdx.x;
+        double acc = 0.0;
+        for (int i = 0; i < ni; i++) acc += atom_mat[(size_t)i * natom + a] * sol[i];
+        part[(size_t)l * NP + VKP_ATOM + a] = acc;
+    }
+    __syncthreads();
+    const double ys2 = ysum;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        ynew[base + i] = sol[i];
+        ymixnew[base + i] = sol[i] / ys2;
+    }
+}
+
+// element sums per column: same per-layer partial + in-order column sum as k_finalize/k_decide
+__global__ void k_atom_layer(int ni, int natom, const double *__restrict__ atom_mat, const double *__restrict__ y,
+                             double *__restrict__ part) {
+    const int l = blockIdx.x;
+    if (threadIdx.x < natom) {
+        const int a = threadIdx.x;
+        double acc = 0.0;
+        for (int i = 0; i < ni; i++) acc += atom_mat[(size_t)i * natom + a] * y[(size_t)l * ni + i];
+        part[(size_t)l * NP + VKP_ATOM + a] = acc;
+    }
+}
+
+__global__ void k_atom_col(int natom, const int *__restrict__ off, const double *__restrict__ part,
+                           double *__restrict__ out) {
+    const int c = blockIdx.x;
+    const int a = threadIdx.x;
+    if (a >= natom) return;
+    double acc = 0.0;
+    for (int l = off[c]; l < off[c + 1]; l++) acc += part[(size_t)l * NP + VKP_ATOM + a];
+    out[(size_t)c * natom + a] = acc;
+}
+
+// per-column decision.  mode 0: only set delta (plain solver); mode 1: one_step logic.
+// flags: 0 retry, 1 accepted, 2 accepted after giving up, 3 inactive; bit 4: run update_mu_dz
+__global__ void k_decide(int natom, const int *__restrict__ off, const double *__restrict__ part, vk_params prm,
+                         int mode, const double *__restrict__ atom_ini, double *__restrict__ atom_loss,
+                         double *__restrict__ atom_loss_prev, vk_colstate *__restrict__ st,
+                         int *__restrict__ flags, double *__restrict__ thist, int tcap) {
+    const int c = blockIdx.x;
+    vk_colstate s = st[c];
+    if (s.done) { if (flags) flags[c] = 3; return; }
+    const int l0 = off[c], l1 = off[c + 1];
+    double delta = -1.0;
+    for (int l = l0; l < l1; l++) delta = fmax(delta, part[(size_t)l * NP + VKP_DELTA]);
+    if (delta < 0) delta = 0.0;
+    s.delta = delta;
+    if (mode == 0) { st[c] = s; return; }
+    int neg = 0;
+    double small = 0.0, nega = 0.0;
+    for (int l = l0; l < l1; l++) {
+        neg |= part[(size_t)l * NP + VKP_NEG] != 0.0;
+        small += part[(size_t)l * NP + VKP_SMALL];
+        nega += part[(size_t)l * NP + VKP_NEGY];
+    }
+    s.small_y += fabs(small);
+    s.nega_y += fabs(nega);
+    double lossmax = 0.0;
+    bool lossnan = false;
+    for (int a = 0; a < natom; a++) {
+        double acc = 0.0;
+        for (int l = l0; l < l1; l++) acc += part[(size_t)l * NP + VKP_ATOM + a];
+        const double ini = atom_ini[(size_t)c * natom + a];
+        const double loss = (acc - ini) / ini;
+        atom_loss[(size_t)c * natom + a] = loss;
+        const double d = fabs(loss - atom_loss_prev[(size_t)c * natom + a]);
+        if (d != d) lossnan = true;
+        lossmax = fmax(lossmax, d);
+    }
+    s.attempts += 1;
+    const bool ok = !neg && !lossnan && lossmax < prm.loss_eps && delta <= prm.rtol;
+    int f;
+    if (ok) {
+        f = 1;
+    } else {
+        if (delta > prm.rtol) s.delta_count += 1;
+        else if (neg) s.nega_count += 1;
+        else s.loss_count += 1;
+        s.dt *= prm.dt_var_min;
+        if (s.dt < prm.dt_min) { s.dt = prm.dt_min; f = 2; }
+        else f = 0;
+    }
+    if (f) {
+        for (int a = 0; a < natom; a++) atom_loss_prev[(size_t)c * natom + a] = atom_loss[(size_t)c * natom + a];
+        if (prm.use_update_mu && s.count % prm.update_frq == 0) f |= 16;
+        // save_step (op.py:1087-1105)
+        s.t += s.dt;
+        s.count += 1;
+        if (thist && s.count - 1 < tcap) thist[(size_t)c * tcap + s.count - 1] = s.t;
+        // step_size (op.py:3105-3125)
+        double h = s.dt;
+        double dl = s.delta;
+        if (dl == 0) dl = 0.01 * prm.rtol;
+        double hf = 0.9 * sqrt(prm.rtol / dl);
+        hf = fmax(hf, prm.dt_var_min);
+        hf = fmin(hf, prm.dt_var_max);
+        h *= hf;
+        h = fmax(h, prm.dt_min);
+        h = fmin(h, prm.dt_max);
+        s.dt = h;
+    }
+    st[c] = s;
+    flags[c] = f;
+}
+
+// update_mu_dz (op.py:944-984) for flagged columns
+__global__ void k_update_mu(int ni, const int *__restrict__ off, const double *__restrict__ ms,
+                            const double *__restrict__ ymix, vk_atm atm, vk_geom geo, const int *__restrict__ flags,
+                            int force, const vk_colstate *__restrict__ st) {
+    const int c = blockIdx.x;
+    if (force) { if (st && st[c].done) return; }
+    else if (!(flags[c] & 16)) return;
+    const int o = off[c], nz = off[c + 1] - o;
+    for (int j = threadIdx.x; j < nz; j += blockDim.x) {
+        const double *ym = ymix + (size_t)(o + j) * ni;
+        double mu = 0.0;
+        for (int i = 0; i < ni; i++) mu += ms[i] * ym[i];
+        geo.mu[o + j] = mu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double gs = geo.gs[c], Rp = geo.Rp[c];
+        const int pref = geo.pref[c];
+        double *g = atm.g + o, *Hp = atm.Hp + o, *dz = geo.dz + o, *zco = geo.zco + o + c;
+        const double *T = atm.Tco + o, *mu = geo.mu + o, *lnp = geo.lnp + o;
+        for (int i = pref; i < nz; i++) {
+            if (i == pref) {
+                g[i] = gs;
+                Hp[i] = KB * T[i] / (mu[i] / NAVO * gs);
+            } else {
+                double q = Rp / (Rp + zco[i]);
+                g[i] = gs * (q * q);
+                Hp[i] = KB * T[i] / (mu[i] / NAVO * g[i]);
+            }
+            dz[i] = Hp[i] * lnp[i];
+            zco[i + 1] = zco[i] + dz[i];
+        }
+        for (int i = pref - 1; i >= 0; i--) {
+            double q = Rp / (Rp + zco[i + 1]);
+            g[i] = gs * (q * q);
+            Hp[i] = KB * T[i] / (mu[i] / NAVO * g[i]);
+            dz[i] = Hp[i] * lnp[i];
+            zco[i] = zco[i + 1] - dz[i];
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nz - 1; j += blockDim.x) {
+        atm.dzi[o + j] = 0.5 * (geo.dz[o + j + 1] + geo.dz[o + j]);
+        atm.Hpi[o + j] = 0.5 * (atm.Hp[o + j] + atm.Hp[o + j + 1]);
+    }
+}
+
+// reset_y (op.py:2524-2534) or renormalise + backup + history (op.py:909-914, 938-942, 1096)
+__global__ void k_apply(int ni, const int *__restrict__ layer_col, const int *__restrict__ flags,
+                        const double *__restrict__ M, const double *__restrict__ ymix, double *y, double *y_prev,
+                        const vk_colstate *__restrict__ st, double *__restrict__ yhist, int H, size_t Lni) {
+    const int l = blockIdx.x;
+    const int c = layer_col[l];
+    const int f = flags[c] & 15;
+    if (f == 3) return;
+    const size_t base = (size_t)l * ni;
+    if (f == 0) {
+        for (int i = threadIdx.x; i < ni; i += blockDim.x) y[base + i] = y_prev[base + i];
+        return;
+    }
+    const double n0 = M[l];
+    double *h = nullptr;
+    if (yhist && H > 0) h = yhist + (size_t)((st[c].count - 1) % H) * Lni + base;
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        double v = n0 * ymix[base + i];
+        y[base + i] = v;
+        y_prev[base + i] = v;
+        if (h) h[i] = v;
+    }
+}
+
+// stop() with conv() (op.py:1018-1085)
+__global__ void __launch_bounds__(256) k_conv(int ni, const int *__restrict__ off, vk_params prm, vk_atm atm,
+                                              const double *__restrict__ y, const double *__restrict__ ymix,
+                                              const int *__restrict__ flags, vk_colstate *__restrict__ st,
+                                              const double *__restrict__ yhist, int H, size_t Lni,
+                                              const double *__restrict__ thist, int tcap) {
+    const int c = blockIdx.x;
+    if (flags && (flags[c] & 15) != 1 && (flags[c] & 15) != 2) return;
+    __shared__ double rv[256];
+    __shared__ int ri[256];
+    __shared__ vk_colstate s;
+    if (threadIdx.x == 0) s = st[c];
+    __syncthreads();
+    if (s.done) return;
+    const int o = off[c], nz = off[c + 1] - o;
+    bool conv = false;
+    const bool test = prm.use_conv && H > 0 && s.t > prm.trun_min && s.count > prm.count_min && s.count <= tcap;
+    if (test) {
+        const int count = s.count;
+        const double x = s.t * prm.st_factor;
+        const int w0 = count - prm.conv_step > 0 ? count - prm.conv_step : 0;
+        double bv = INFINITY;
+        int bi = 0x7fffffff;
+        for (int q = w0 + threadIdx.x; q < count; q += blockDim.x) {
+            double v = fabs(thist[(size_t)c * tcap + q] - x);
+            if (v < bv || (v == bv && q < bi)) { bv = v; bi = q; }
+        }
+        rv[threadIdx.x] = bv;
+        ri[threadIdx.x] = bi;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                double ov = rv[threadIdx.x + w];
+                int oi = ri[threadIdx.x + w];
+                if (ov < rv[threadIdx.x] || (ov == rv[threadIdx.x] && oi < ri[threadIdx.x])) {
+                    rv[threadIdx.x] = ov;
+                    ri[threadIdx.x] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        int indx = ri[0];
+        if (indx == count - 1) indx -= 1;
+        if (indx < count - prm.conv_step) indx = count - prm.conv_step;
+        if (indx < 0) indx += count;   // Python negative index
+        const double *ycur = y;        // y_time[count-1] is the renormalised current y
+        const double *yold = yhist + (size_t)(indx % H) * Lni;
+        double lmax = -1.0;
+        for (int q = threadIdx.x; q < nz * ni; q += blockDim.x) {
+            const int j = q / ni;
+            const size_t e = (size_t)(o + j) * ni + (q - j * ni);
+            double d = fabs((ycur[e] - yold[e]) / atm.M[o + j]);
+            const double mx = ymix[e];
+            if (mx < prm.mtol_conv) d = 0;
+            if (ycur[e] < prm.atol) d = 0;
+            if (mx > 0) lmax = fmax(lmax, d / mx);
+        }
+        __syncthreads();
+        rv[threadIdx.x] = lmax;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) rv[threadIdx.x] = fmax(rv[threadIdx.x], rv[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double longdy = rv[0];
+            double longdydt = longdy / (thist[(size_t)c * tcap + count - 1] - thist[(size_t)c * tcap + indx]);
+            double smin = 1e300;
+            for (int j = 0; j < nz - 1; j++) {
+                double hq = 0.1 * atm.Hp[o + j];
+                smin = fmin(smin, atm.Kzz[o + j] / (hq * hq));
+            }
+            smin = fmin(smin, 1.e-8);
+            smin = fmax(smin, 1.e-10);
+            s.longdy = longdy;
+            s.longdydt = longdydt;
+            conv = ((longdy < prm.yconv_cri && longdydt < prm.slope_cri) ||
+                    (longdy < prm.yconv_min && longdydt < smin)) && 0.0 < prm.flux_cri;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (conv) { s.end_case = 1; s.done = 1; }
+        else if (s.t > prm.runtime) { s.end_case = 2; s.done = 1; }
+        else if (s.count > prm.count_max) { s.end_case = 3; s.done = 1; }
+        st[c] = s;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan / columns
+// ---------------------------------------------------------------------------
+
+extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
+    if (!n || !out || n->ni <= 0 || n->npairs <= 0) return VK_EINVAL;
+    if (n->ni > VK_NI_MAX || n->natom > VK_NATOM_MAX) return VK_ETOOBIG;
+    vk_plan *p = new vk_plan();
+    p->ni = n->ni; p->npairs = n->npairs; p->nr = 2 * n->npairs; p->natom = n->natom;
+    p->nchem = n->nchem; p->nslot = n->nslot; p->njnz = n->njnz; p->njac = n->njac;
+    int r = 0;
+    auto &v = p->allocs;
+    r = r ? r : upload(v, &p->reac, n->reac, (size_t)4 * n->npairs);
+    r = r ? r : upload(v, &p->prod, n->prod, (size_t)4 * n->npairs);
+    r = r ? r : upload(v, &p->kind, n->kind, (size_t)n->npairs);
+    r = r ? r : upload(v, &p->rev, n->rev, (size_t)n->npairs);
+    r = r ? r : upload(v, &p->par, n->par, (size_t)8 * n->npairs);
+    r = r ? r : upload(v, &p->nasa9, n->nasa9, (size_t)20 * n->ni);
+    r = r ? r : upload(v, &p->atom_mat, n->atom_mat, (size_t)n->ni * n->natom);
+    r = r ? r : upload(v, &p->ms, n->ms, (size_t)n->ni);
+    r = r ? r : upload(v, &p->alpha, n->alpha, (size_t)n->ni);
+    r = r ? r : upload(v, &p->chem_ptr, n->chem_ptr, (size_t)n->ni + 1);
+    r = r ? r : upload(v, &p->chem_pair, n->chem_pair, (size_t)n->nchem);
+    r = r ? r : upload(v, &p->chem_sign, n->chem_sign, (size_t)n->nchem);
+    r = r ? r : upload(v, &p->slot_pair, n->slot_pair, (size_t)n->nslot);
+    r = r ? r : upload(v, &p->slot_sp, n->slot_sp, (size_t)n->nslot);
+    r = r ? r : upload(v, &p->jac_map, n->jac_map, (size_t)n->ni * n->ni);
+    r = r ? r : upload(v, &p->jac_ptr, n->jac_ptr, (size_t)n->njnz + 1);
+    r = r ? r : upload(v, &p->jac_slot, n->jac_slot, (size_t)n->njac);
+    r = r ? r : upload(v, &p->jac_coef, n->jac_coef, (size_t)n->njac);
+    if (r) { vk_plan_destroy(p); return r; }
+    *out = p;
+    return 0;
+}
+
+extern "C" int vk_plan_destroy(vk_plan *p) {
+    if (!p) return 0;
+    for (void *q : p->allocs) (void)hipFree(q);
+    delete p;
+    return 0;
+}
+
+extern "C" int vk_plan_info(const vk_plan *p, int *ni, int *nr, int *max_ni) {
+    if (ni) *ni = p ? p->ni : 0;
+    if (nr) *nr = p ? p->nr : 0;
+    if (max_ni) *max_ni = VK_NI_MAX;
+    return 0;
+}
+
+extern "C" int vk_columns_create(int ncols, const int *nz, vk_columns **out) {
+    if (ncols <= 0 || !nz || !out) return VK_EINVAL;
+    vk_columns *c = new vk_columns();
+    c->ncols = ncols;
+    c->off.resize(ncols + 1);
+    c->off[0] = 0;
+    c->max_nz = 0;
+    for (int i = 0; i < ncols; i++) {
+        if (nz[i] < 2) { delete c; return VK_EINVAL; }
+        c->off[i + 1] = c->off[i] + nz[i];
+        c->max_nz = nz[i] > c->max_nz ? nz[i] : c->max_nz;
+    }
+    c->L = c->off[ncols];
+    std::vector<int> lc(c->L);
+    for (int i = 0; i < ncols; i++)
+        for (int l = c->off[i]; l < c->off[i + 1]; l++) lc[l] = i;
+    if (hipMalloc((void **)&c->d_off, sizeof(int) * (ncols + 1)) != hipSuccess) { delete c; return VK_ENOMEM; }
+    if (hipMalloc((void **)&c->d_layer_col, sizeof(int) * c->L) != hipSuccess) {
+        (void)hipFree(c->d_off); delete c; return VK_ENOMEM;
+    }
+    VK_CHECK(hipMemcpy(c->d_off, c->off.data(), sizeof(int) * (ncols + 1), hipMemcpyHostToDevice));
+    VK_CHECK(hipMemcpy(c->d_layer_col, lc.data(), sizeof(int) * c->L, hipMemcpyHostToDevice));
+    *out = c;
+    return 0;
+}
+
+extern "C" int vk_columns_destroy(vk_columns *c) {
+    if (!c) return 0;
+    (void)hipFree(c->d_off);
+    (void)hipFree(c->d_layer_col);
+    delete c;
+    return 0;
+}
+
+extern "C" int vk_columns_layers(const vk_columns *c, int *nlayers) {
+    if (!c || !nlayers) return VK_EINVAL;
+    *nlayers = c->L;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// integrator
+// ---------------------------------------------------------------------------
+
+extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols, int history, vk_integrator **out) {
+    if (!plan || !cols || !out || history < 0) return VK_EINVAL;
+    vk_integrator *it = new vk_integrator();
+    it->plan = plan;
+    it->cols = cols;
+    it->history = history;
+    it->tcap = 0;
+    const size_t L = cols->L, ni = plan->ni, nc = cols->ncols;
+    auto &v = it->allocs;
+    int r = 0;
+    r = r ? r : dalloc(v, &it->D, L * ni * ni);
+    r = r ? r : dalloc(v, &it->Dinv, L * ni * ni);
+    r = r ? r : dalloc(v, &it->Lsub, L * ni);
+    r = r ? r : dalloc(v, &it->Usup, L * ni);
+    r = r ? r : dalloc(v, &it->b1, L * ni);
+    r = r ? r : dalloc(v, &it->x1, L * ni);
+    r = r ? r : dalloc(v, &it->b2, L * ni);
+    r = r ? r : dalloc(v, &it->x2, L * ni);
+    r = r ? r : dalloc(v, &it->work, L * ni);
+    r = r ? r : dalloc(v, &it->ztmp, L * ni);
+    r = r ? r : dalloc(v, &it->yk2, L * ni);
+    r = r ? r : dalloc(v, &it->part, L * NP);
+    r = r ? r : dalloc(v, &it->info, nc);
+    r = r ? r : dalloc(v, &it->flags, nc);
+    r = r ? r : dalloc(v, &it->colscr, nc * 16);
+    if (!r && history > 0) {
+        r = dalloc(v, &it->yhist, (size_t)history * L * ni);
+        it->tcap = 0;
+    }
+    if (r) { vk_integrator_destroy(it); return r; }
+    if (hipEventCreate(&it->ev0) != hipSuccess || hipEventCreate(&it->ev1) != hipSuccess) {
+        vk_integrator_destroy(it);
+        return VK_ENOMEM;
+    }
+    it->timing = 0;
+    it->factor_ms = 0.0;
+    it->factor_launches = 0;
+    *out = it;
+    return 0;
+}
+
+static int ensure_thist(vk_integrator *it, int count_max) {
+    const int need = count_max + 2;
+    if (it->history <= 0 || it->tcap >= need) return 0;
+    double *t = nullptr;
+    if (hipMalloc((void **)&t, sizeof(double) * (size_t)it->cols->ncols * need) != hipSuccess) return VK_ENOMEM;
+    VK_CHECK(hipMemset(t, 0, sizeof(double) * (size_t)it->cols->ncols * need));
+    if (it->thist) {
+        // keep the recorded times
+        for (int c = 0; c < it->cols->ncols; c++)
+            VK_CHECK(hipMemcpy(t + (size_t)c * need, it->thist + (size_t)c * it->tcap, sizeof(double) * it->tcap,
+                               hipMemcpyDeviceToDevice));
+        (void)hipFree(it->thist);
+    }
+    it->thist = t;
+    it->tcap = need;
+    return 0;
+}
+
+extern "C" int vk_integrator_destroy(vk_integrator *it) {
+    if (!it) return 0;
+    for (void *q : it->allocs) (void)hipFree(q);
+    if (it->thist) (void)hipFree(it->thist);
+    if (it->ev0) (void)hipEventDestroy(it->ev0);
+    if (it->ev1) (void)hipEventDestroy(it->ev1);
+    delete it;
+    return 0;
+}
+
+extern "C" int vk_integrator_set_timing(vk_integrator *it, int on) {
+    if (!it) return VK_EINVAL;
+    it->timing = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int vk_integrator_stats(const vk_integrator *it, double *factor_ms, long long *launches) {
+    if (!it) return VK_EINVAL;
+    if (factor_ms) *factor_ms = it->factor_ms;
+    if (launches) *launches = it->factor_launches;
+    return 0;
+}
+
+// one Ros2 attempt on all active columns (op.py:2890-2932): k1 in it->x1, k2 in it->x2
+static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *prm, const double *k, double *y,
+                        double *ymix, vk_colstate *st, int clip, bool timed, hipStream_t s) {
+    const vk_plan *p = it->plan;
+    const vk_columns *cl = it->cols;
+    const int ni = p->ni;
+    const size_t L = cl->L;
+    // dt per column lives in the state struct; gather it contiguously
+    double *dt = it->colscr;
+    hipLaunchKernelGGL(k_gather_dt, dim3((cl->ncols + 127) / 128), dim3(128), 0, s, cl->ncols, st, dt);
+    int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->b1, s);
+    if (r) return r;
+    r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->D, it->Lsub, it->Usup, s);
+    if (r) return r;
+    if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
+    r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, s);
+    if (r) return r;
+    if (timed) VK_CHECK(hipEventRecord(it->ev1, s));
+    r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b1, it->x1, it->work, it->ztmp, prm->refine,
+                                st, s);   // k1
+    if (r) return r;
+    const int n = (int)(L * ni);
+    hipLaunchKernelGGL(k_yk2, dim3((n + 255) / 256), dim3(256), 0, s, n, ni, cl->d_layer_col, y, it->x1, st, it->yk2);
+    r = vk_launch_rhs(p, cl, atm, it->yk2, k, 3, prm->use_moldiff, it->x1, dt, st, it->b2, s);   // f(yk2) - 2/(r h) k1
+    if (r) return r;
+    r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b2, it->x2, it->work, it->ztmp, prm->refine,
+                                st, s);   // k2
+    if (r) return r;
+    hipLaunchKernelGGL(k_finalize, dim3(cl->L), dim3(256), 0, s, ni, p->natom, cl->d_off, cl->d_layer_col,
+                       p->atom_mat, y, ymix, it->x1, it->x2, it->yk2, *prm, clip, st, y, ymix, it->part);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+static void accumulate_factor_time(vk_integrator *it) {
+    float ms = 0.f;
+    if (hipEventSynchronize(it->ev1) == hipSuccess && hipEventElapsedTime(&ms, it->ev0, it->ev1) == hipSuccess) {
+        it->factor_ms += ms;
+        it->factor_launches += 1;
+    }
+}
+
+extern "C" int vk_ros2_solver(vk_integrator *it, const vk_atm *atm, const vk_params *prm, const double *d_k,
+                              double *d_y, double *d_ymix, vk_colstate *d_st, void *stream) {
+    if (!it || !atm || !prm || !d_k || !d_y || !d_ymix || !d_st) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    int r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 0, false, s);
+    if (r) return r;
+    hipLaunchKernelGGL(k_decide, dim3(it->cols->ncols), dim3(1), 0, s, it->plan->natom, it->cols->d_off, it->part,
+                       *prm, 0, nullptr, nullptr, nullptr, d_st, nullptr, nullptr, 0);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom, const vk_params *prm,
+                            const double *d_k, double *d_y, double *d_ymix, double *d_y_prev, double *d_atom_ini,
+                            double *d_atom_loss, double *d_atom_loss_prev, vk_colstate *d_st, int n_attempts,
+                            void *stream) {
+    if (!it || !atm || !prm || !d_k || !d_y || !d_ymix || !d_y_prev || !d_atom_ini || !d_atom_loss ||
+        !d_atom_loss_prev || !d_st || n_attempts < 0)
+        return VK_EINVAL;
+    if (prm->use_update_mu && !geom) return VK_EINVAL;
+    if (prm->use_conv && it->history > 0 && it->history < prm->conv_step + 1) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const vk_plan *p = it->plan;
+    const vk_columns *cl = it->cols;
+    int r = ensure_thist(it, prm->count_max);
+    if (r) return r;
+    vk_geom g{};
+    if (geom) g = *geom;
+    for (int a = 0; a < n_attempts; a++) {
+        r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, it->timing && a == n_attempts - 1, s);
+        if (r) return r;
+        hipLaunchKernelGGL(k_decide, dim3(cl->ncols), dim3(1), 0, s, p->natom, cl->d_off, it->part, *prm, 1,
+                           d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->flags, it->thist, it->tcap);
+        if (prm->use_update_mu)
+            hipLaunchKernelGGL(k_update_mu, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, p->ms, d_ymix,
+                               *atm, g, it->flags, 0, d_st);
+        hipLaunchKernelGGL(k_apply, dim3(cl->L), dim3(128), 0, s, p->ni, cl->d_layer_col, it->flags, atm->M, d_ymix,
+                           d_y, d_y_prev, d_st, it->yhist, it->history, (size_t)cl->L * p->ni);
+        hipLaunchKernelGGL(k_conv, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, *prm, *atm, d_y, d_ymix,
+                           it->flags, d_st, it->yhist, it->history, (size_t)cl->L * p->ni, it->thist, it->tcap);
+        VK_CHECK(hipGetLastError());
+    }
+    if (n_attempts > 0 && it->timing) accumulate_factor_time(it);
+    return 0;
+}
+
+extern "C" int vk_update_mu_dz(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const vk_geom *geom,
+                               const double *d_ymix, const vk_colstate *d_st, int force, void *stream) {
+    if (!plan || !cols || !atm || !geom || !d_ymix) return VK_EINVAL;
+    if (!force) return VK_EINVAL;
+    hipLaunchKernelGGL(k_update_mu, dim3(cols->ncols), dim3(256), 0, (hipStream_t)stream, plan->ni, cols->d_off,
+                       plan->ms, d_ymix, *atm, *geom, nullptr, 1, d_st);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vk_atom_sums(const vk_plan *plan, const vk_columns *cols, const double *d_y, double *d_atom_sum,
+                            void *stream) {
+    if (!plan || !cols || !d_y || !d_atom_sum) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    double *part = nullptr;
+    VK_CHECK(hipMallocAsync((void **)&part, sizeof(double) * (size_t)cols->L * NP, s));
+    hipLaunchKernelGGL(k_atom_layer, dim3(cols->L), dim3(64), 0, s, plan->ni, plan->natom, plan->atom_mat, d_y, part);
+    hipLaunchKernelGGL(k_atom_col, dim3(cols->ncols), dim3(64), 0, s, plan->natom, cols->d_off, part, d_atom_sum);
+    VK_CHECK(hipGetLastError());
+    VK_CHECK(hipFreeAsync(part, s));
+    return 0;
+}

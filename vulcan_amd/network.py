@@ -266,7 +266,9 @@ def synthetic_network(ni: int, n_bi: int, n_tb: int = 0, n_photo: int = 0,
         must.append(tuple(v2))
     # small molecules first: fill by total atom count, random within a level,
     # so that many element-conserving reactions exist among the species
-    rest = [v for v in pool if v not in must]
+    # pure-hydrogen species stop at H2, so that H2 is the bulk gas under
+    # hydrogen-dominated element abundances
+    rest = [v for v in pool if v not in must and not (v[0] > 2 and sum(v) == v[0])]
     keys = rng.random(len(rest))
     rest = [v for _, v in sorted(zip(keys, rest), key=lambda t: (sum(t[1]), t[0]))]
     n_chem = ni - 1

@@ -1,0 +1,241 @@
+"""``op.Ros2`` / ``op.Integration`` counterparts for VULCAN-style host code.
+
+The reference selects its solver by name (``getattr(op, vulcan_cfg.ode_solver)()``,
+vulcan.py:162-163) and drives it from ``Integration.__call__`` (op.py:808-936)
+through ``naming_solver``, ``one_step``, ``step_size`` and ``solver``
+(op.py:3078-3125).  This module offers the same methods with the same
+argument meaning on the same ``var``/``atm``/``para`` attribute names
+(store.py:21-208):
+
+* ``Ros2.solver(var, atm, para)``: one attempt on the GPU; replaces ``var.y``
+  and ``var.ymix`` and sets ``para.delta`` exactly as op.py:2860-3007;
+* ``Ros2.one_step`` / ``clip`` / ``loss`` / ``step_ok`` / ``step_reject`` /
+  ``reset_y`` / ``step_size``: the reference's host logic around the solver
+  (op.py:2447-2534, 3091-3125), kept on the host for this plain path;
+* ``Integration(solver)(var, atm, para, make_atm)``: the whole loop
+  (backup, one_step, update_mu_dz, renormalisation, save_step, step_size,
+  stop/conv) resident on the device, polled every ``poll`` attempts.
+
+Photochemical radiative transfer (``compute_tau/flux/J``), condensation,
+ions and flux boundary conditions are not part of the accelerated path and
+raise ``NotImplementedError``; photolysis rates enter as prescribed ``k``.
+Numerical failure is never raised (negative y, delta and element loss are
+reported through ``var``/``para``, as in the reference).
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from .atmosphere import Column
+from .chem_funs import ChemFuns, k_dict_to_array
+from .config import Config
+from .network import Network
+
+
+def _column_from_atm(atm, nz: int, ni: int) -> Column:
+    g = lambda n, d=None: np.asarray(getattr(atm, n), dtype=np.float64) if hasattr(atm, n) else d
+    return Column(pco=g('pco', np.zeros(nz)), pico=g('pico', np.ones(nz + 1)), Tco=g('Tco'), Kzz=g('Kzz'),
+                  vz=g('vz', np.zeros(nz - 1)), M=g('M'), Dzz=g('Dzz', np.zeros((nz - 1, ni))),
+                  alpha=g('alpha', np.zeros(ni)), ms=g('ms'), mu=g('mu', np.zeros(nz)), g=g('g'), Hp=g('Hp'),
+                  dz=g('dz', np.zeros(nz)), zco=g('zco', np.zeros(nz + 1)), dzi=g('dzi'),
+                  Ti=g('Ti', np.zeros(nz - 1)), Hpi=g('Hpi', np.zeros(nz - 1)), gs=float(getattr(atm, 'gs', 0.0)),
+                  Rp=float(getattr(atm, 'Rp', 1.0)), pref_indx=int(getattr(atm, 'pref_indx', 0)),
+                  tau=g('tau', None))
+
+
+class Ros2:
+    """op.Ros2 counterpart (op.py:2824-3125) for one network."""
+
+    def __init__(self, net: Network, cfg: Optional[Config] = None):
+        self.net = net
+        self.cfg = cfg or Config()
+        self.cfg.check_supported()
+        self.chem = ChemFuns(net)
+        self.atom_list = net.atoms()
+        self._atom_mat = net.atom_matrix()
+        self._batch = None
+        self._batch_nz = None
+
+    # -- plugin surface --------------------------------------------------------
+    def naming_solver(self, para):
+        """op.py:3078-3088."""
+        para.solver_str = 'solver'
+
+    def compute_tau(self, var, atm):
+        raise NotImplementedError('photochemical radiative transfer is outside the accelerated path; '
+                                  'prescribe photolysis rates in var.k')
+
+    compute_flux = compute_J = compute_Jion = compute_tau
+
+    # -- the device attempt ------------------------------------------------------
+    def _device_batch(self, atm, nz):
+        import torch
+        from .device import ColumnBatch
+        col = _column_from_atm(atm, nz, self.net.ni)
+        if self._batch is None or self._batch_nz != nz:
+            from .device import DevicePlan
+            plan = DevicePlan(self.net, alpha=col.alpha if self.cfg.use_moldiff else None)
+            y0 = np.ones((nz, self.net.ni))
+            self._batch = ColumnBatch(plan, [col], [y0], self.cfg, k=np.zeros((nz, self.net.nr)))
+            self._batch_nz = nz
+        b = self._batch
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64))
+        pad = lambda v: np.append(np.asarray(v, dtype=np.float64), 0.0)
+        for name in ('Tco', 'M', 'g', 'Hp'):
+            getattr(b, name).copy_(T(getattr(col, name)))
+        for name in ('Kzz', 'dzi', 'Ti', 'Hpi'):
+            getattr(b, name).copy_(T(pad(getattr(col, name))))
+        Dz = np.zeros((nz, self.net.ni))
+        Dz[:nz - 1] = col.Dzz
+        b.Dzz.copy_(T(Dz))
+        return b
+
+    def solver(self, var, atm, para):
+        """One Ros2 attempt (op.py:2860-3007) on the GPU."""
+        import torch
+        from . import _lib
+        y = np.ascontiguousarray(var.y, dtype=np.float64)
+        nz = y.shape[0]
+        b = self._device_batch(atm, nz)
+        b.k.copy_(torch.as_tensor(k_dict_to_array(var.k, nz, self.net.nr)))
+        b.y.copy_(torch.as_tensor(y))
+        b.ymix.copy_(torch.as_tensor(np.ascontiguousarray(var.ymix, dtype=np.float64)))
+        st = np.zeros(1, _lib.COLSTATE_DTYPE)
+        st['dt'] = var.dt
+        b.set_host_state(st)
+        b.attempt()
+        var.y = b.y.cpu().numpy()
+        var.ymix = b.ymix.cpu().numpy()
+        para.delta = float(b.host_state()['delta'][0])
+        return var, para
+
+    # -- host logic around the solver (op.py:2447-2534) ------------------------------
+    def clip(self, var, para, atm, pos_cut=None, nega_cut=None):
+        cfg = self.cfg
+        pos_cut = cfg.pos_cut if pos_cut is None else pos_cut
+        nega_cut = cfg.nega_cut if nega_cut is None else nega_cut
+        y, ymix = var.y, var.ymix.copy()
+        para.small_y += np.abs(np.sum(y[np.logical_and(y < pos_cut, y >= 0)]))
+        para.nega_y += np.abs(np.sum(y[np.logical_and(y > nega_cut, y <= 0)]))
+        y[np.logical_and(y < pos_cut, y >= nega_cut)] = 0.
+        y[np.logical_and(ymix < cfg.mtol, y < 0)] = 0.
+        var = self.loss(var)
+        var.y, var.ymix = y, y / np.vstack(np.sum(y, axis=1))
+        return var, para
+
+    def loss(self, var):
+        y = var.y
+        for a, atom in enumerate(self.atom_list):
+            var.atom_sum[atom] = np.sum([self._atom_mat[i, a] * y[:, i] for i in range(y.shape[1])])
+            var.atom_loss[atom] = (var.atom_sum[atom] - var.atom_ini[atom]) / var.atom_ini[atom]
+        return var
+
+    def step_ok(self, var, para, loss_eps=None, rtol=None):
+        loss_eps = self.cfg.loss_eps if loss_eps is None else loss_eps
+        rtol = self.cfg.rtol if rtol is None else rtol
+        cur = np.fromiter((var.atom_loss[a] for a in self.atom_list), float)
+        prev = np.fromiter((var.atom_loss_prev[a] for a in self.atom_list), float)
+        return bool(np.all(var.y >= 0) and np.amax(np.abs(cur - prev)) < loss_eps and para.delta <= rtol)
+
+    def step_reject(self, var, para, loss_eps=None, rtol=None):
+        rtol = self.cfg.rtol if rtol is None else rtol
+        if para.delta > rtol:
+            para.delta_count += 1
+        elif np.any(var.y < 0):
+            para.nega_count += 1
+        else:
+            para.loss_count += 1
+        var = self.reset_y(var)
+        if var.dt < self.cfg.dt_min:
+            var.dt = self.cfg.dt_min
+            var.y[var.y < 0] = 0.
+            return True
+        return False
+
+    def reset_y(self, var, dt_reduc=None):
+        var.y = var.y_prev
+        var.dt *= self.cfg.dt_var_min if dt_reduc is None else dt_reduc
+        return var
+
+    def one_step(self, var, atm, para):
+        """op.py:3091-3103."""
+        while True:
+            var, para = getattr(self, para.solver_str or 'solver')(var, atm, para)
+            var, para = self.clip(var, para, atm)
+            if self.step_ok(var, para):
+                break
+            elif self.step_reject(var, para):
+                break
+        return var, para
+
+    def step_size(self, var, para, dt_var_min=None, dt_var_max=None, dt_min=None, dt_max=None):
+        """op.py:3105-3125."""
+        cfg = self.cfg
+        h = var.dt
+        delta = para.delta
+        rtol = cfg.rtol
+        if delta == 0:
+            delta = 0.01 * rtol
+        h_factor = 0.9 * (rtol / delta) ** 0.5
+        h_factor = np.maximum(h_factor, cfg.dt_var_min if dt_var_min is None else dt_var_min)
+        h_factor = np.minimum(h_factor, cfg.dt_var_max if dt_var_max is None else dt_var_max)
+        h *= h_factor
+        h = np.maximum(h, cfg.dt_min if dt_min is None else dt_min)
+        h = np.minimum(h, cfg.dt_max if dt_max is None else dt_max)
+        var.dt = float(h)
+        return var
+
+
+class Integration:
+    """Integration.__call__ (op.py:808-936) resident on the device.
+
+    ``var`` must carry ``y``, ``ymix``, ``k``, ``dt``, ``t`` and the element
+    dicts; ``atm`` the column arrays.  On return ``var.y``, ``var.ymix``,
+    ``var.t``, ``var.dt``, ``var.longdy``, ``var.longdydt``, ``var.t_time``,
+    ``var.y_time`` (the last ``conv_step`` states), ``para.count``,
+    ``para.end_case`` and the reject counters are set.
+    """
+
+    def __init__(self, odesolver: Ros2, output=None, poll: int = 64):
+        self.odesolver = odesolver
+        self.output = output
+        self.poll = poll
+
+    def __call__(self, var, atm, para, make_atm=None):
+        import torch
+        from .device import ColumnBatch, DevicePlan
+        solver = self.odesolver
+        cfg = solver.cfg
+        y = np.ascontiguousarray(var.y, dtype=np.float64)
+        nz, ni = y.shape
+        col = _column_from_atm(atm, nz, ni)
+        plan = DevicePlan(solver.net, alpha=col.alpha if cfg.use_moldiff else None)
+        H = cfg.conv_step + 1
+        b = ColumnBatch(plan, [col], [y], cfg, history=H, k=k_dict_to_array(var.k, nz, solver.net.nr))
+        b.ymix.copy_(torch.as_tensor(np.ascontiguousarray(var.ymix, dtype=np.float64)))
+        ai = np.array([var.atom_ini[a] for a in solver.atom_list])
+        b.atom_ini.copy_(torch.as_tensor(ai[None]))
+        st = b.host_state()
+        st['dt'], st['t'], st['count'] = var.dt, var.t, getattr(para, 'count', 0)
+        b.set_host_state(st)
+        while True:
+            b.integrate(self.poll)
+            h = b.host_state()[0]
+            if h['done']:
+                break
+        var.y, var.ymix = b.y.cpu().numpy(), b.ymix.cpu().numpy()
+        var.t, var.dt = float(h['t']), float(h['dt'])
+        var.longdy, var.longdydt = float(h['longdy']), float(h['longdydt'])
+        para.count, para.end_case, para.delta = int(h['count']), int(h['end_case']), float(h['delta'])
+        para.nega_count, para.loss_count, para.delta_count = int(h['nega_count']), int(h['loss_count']), \
+            int(h['delta_count'])
+        for name in ('g', 'Hp'):
+            setattr(atm, name, b.column(name, 0))
+        for name in ('dzi', 'Hpi'):
+            setattr(atm, name, b.column(name, 0)[:nz - 1])
+        loss = b.atom_loss.cpu().numpy()[0]
+        for a, atom in enumerate(solver.atom_list):
+            var.atom_loss[atom] = float(loss[a])
+        return var

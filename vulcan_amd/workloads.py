@@ -15,8 +15,11 @@
                (93, 600), (150, 1100; three extra elements); nz uniform on
                [100, 150].
 
-Initial mixing ratios are log-uniform on [1e-12, 1e-1], normalised to 1, and
-``y = n_0 * ymix``.
+``small`` starts from mixing ratios log-uniform on [1e-12, 1e-1], normalised
+to 1.  The others start, as VULCAN does by default (ini_mix = 'EQ'), from
+chemical equilibrium at each layer's T and P (vulcan_amd/equilibrium.py) for
+the element abundances of vulcan_cfg.py:30-34 (O/H, C/H, N/H, He/H; extra
+elements at the S/H value).  ``y = n_0 * ymix``.
 """
 from __future__ import annotations
 
@@ -27,7 +30,17 @@ import numpy as np
 
 from .atmosphere import Column, build_column, initial_mixing
 from .config import Config
+from .equilibrium import equilibrium
 from .network import Network, synthetic_network
+
+# vulcan_cfg.py:30-34 (relative to H)
+SOLAR = {'H': 1.0, 'O': 5.37E-4, 'C': 2.95E-4, 'N': 7.08E-5, 'He': 0.0838, 'S': 1.41E-5,
+         'X': 1.41E-5, 'Y': 1.41E-5, 'Z': 1.41E-5}
+
+
+def element_vector(net: Network, abund=None) -> np.ndarray:
+    abund = dict(SOLAR, **(abund or {}))
+    return np.array([abund.get(e, 0.0) for e in net.elements])
 
 SEED_SMALL, SEED_FULL, SEED_BATCH, SEED_MIXED = 1001, 1002, 1003, 1004
 
@@ -54,8 +67,16 @@ def mixed_networks() -> List[Network]:
             synthetic_network(150, 935, 165, 0, seed=SEED_MIXED + 2, extra_elements=3)]
 
 
-def _case(net, nz, T_bot, T_top, Kzz_bot, Kzz_top, rng, cfg, tau=False, T_profile=None):
-    ymix0 = initial_mixing(net.ni, nz, rng)
+def _case(net, nz, T_bot, T_top, Kzz_bot, Kzz_top, rng, cfg, tau=False, T_profile=None, start='eq',
+          abund=None):
+    if start == 'random':
+        ymix0 = initial_mixing(net.ni, nz, rng)
+    else:
+        Tco = np.array(T_profile, dtype=np.float64) if T_profile is not None else np.linspace(T_bot, T_top, nz)
+        pco = np.logspace(np.log10(1e9), np.log10(1e4), nz)
+        ymix0 = equilibrium(net, Tco, pco, element_vector(net, abund))
+        ymix0 = np.maximum(ymix0, 1e-30)
+        ymix0 = ymix0 / ymix0.sum(axis=1, keepdims=True)
     col = build_column(net, nz, T_bot, T_top, P_b=1e9, P_t=1e4, Kzz_bot=Kzz_bot,
                        Kzz_top=Kzz_top, ymix0=ymix0, cfg=cfg,
                        tau_top=1e-4 if tau else None, tau_bot=1e2 if tau else None,
@@ -67,7 +88,7 @@ def _case(net, nz, T_bot, T_top, Kzz_bot, Kzz_top, rng, cfg, tau=False, T_profil
 def small_case(nz: int = 60, use_moldiff: bool = True, net: Optional[Network] = None) -> ColumnCase:
     cfg = Config(use_moldiff=use_moldiff)
     rng = np.random.default_rng(np.random.SeedSequence(SEED_SMALL).spawn(1)[0])
-    return _case(net or small_network(), nz, 2000., 500., 1e7, None, rng, cfg)
+    return _case(net or small_network(), nz, 2000., 500., 1e7, None, rng, cfg, start='random')
 
 
 def full_case(nz: int = 150, net: Optional[Network] = None) -> ColumnCase:
