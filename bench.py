@@ -11,7 +11,9 @@ resident on the device before timing starts.
 
 The JSON line also carries
 * ``roofline`` for the dominant kernel (block-Thomas factorisation), with the
-  average launch time measured by HIP events on the launch stream;
+  average launch time measured by HIP events on the launch stream (eager
+  attempts right after the timed region; the timed region itself replays
+  the captured per-attempt HIP graph);
 * ``cpu_baseline``: the CPU oracle (reference algorithm: dense lhs, band
   repack, two banded LAPACK solves per attempt) timed on the host in a
   single-threaded subprocess on a bounded sample;
@@ -47,51 +49,61 @@ def run_steps(batch, nsteps, col=0):
     return attempts
 
 
-def bench_single(steps, warmup):
+def factor_timing(b, attempts):
+    """Average factor-kernel launch time over `attempts` eager attempts,
+    from HIP events recorded on the launch stream."""
+    b.set_graph(False)
+    b.set_timing(True)
+    ms0, n0 = b.factor_stats()
+    for _ in range(attempts):
+        b.integrate(1, use_conv=False)
+    _sync()
+    ms1, n1 = b.factor_stats()
+    b.set_timing(False)
+    return (ms1 - ms0) / max(n1 - n0, 1)
+
+
+def bench_single(steps, warmup, graph=True):
     import numpy as np
     from vulcan_amd import workloads as W
     from vulcan_amd.device import ColumnBatch, DevicePlan
     c = W.full_case()
     plan = DevicePlan(c.net)
-    b = ColumnBatch(plan, [c.col], [c.y0], c.cfg)
+    b = ColumnBatch(plan, [c.col], [c.y0], c.cfg, graph=graph)
     _sync()
     run_steps(b, warmup)
     _sync()
-    b.set_timing(True)
-    ms0, n0 = b.factor_stats()
     t0 = time.perf_counter()
     attempts = run_steps(b, steps)
     _sync()
     dt = time.perf_counter() - t0
-    ms1, n1 = b.factor_stats()
     st = b.host_state()[0]
+    factor_ms = factor_timing(b, 20)
     nz, ni = c.col.nz, c.net.ni
     return dict(time=dt, steps=steps, attempts=attempts, nz=nz, ni=ni, npairs=c.net.npairs,
-                factor_ms=(ms1 - ms0) / max(n1 - n0, 1), t_model=float(st['t']), dt_model=float(st['dt']))
+                factor_ms=factor_ms, t_model=float(st['t']), dt_model=float(st['dt']))
 
 
-def bench_batched(attempts, warmup, ncols=256):
+def bench_batched(attempts, warmup, ncols=256, graph=True):
     import numpy as np
     from vulcan_amd import workloads as W
     from vulcan_amd.device import ColumnBatch, DevicePlan
     cases = W.batched_cases(ncols)
     plan = DevicePlan(cases[0].net)
-    b = ColumnBatch(plan, [c.col for c in cases], [c.y0 for c in cases], cases[0].cfg)
+    b = ColumnBatch(plan, [c.col for c in cases], [c.y0 for c in cases], cases[0].cfg, graph=graph)
     _sync()
     b.integrate(warmup, use_conv=False)
     _sync()
     c0 = b.host_state()['count'].astype(np.int64).sum()
-    b.set_timing(True)
-    ms0, n0 = b.factor_stats()
     t0 = time.perf_counter()
     b.integrate(attempts, use_conv=False)
     _sync()
     dt = time.perf_counter() - t0
-    ms1, n1 = b.factor_stats()
     c1 = b.host_state()['count'].astype(np.int64).sum()
+    factor_ms = factor_timing(b, 3)
     nzs = np.array([c.col.nz for c in cases])
     return dict(time=dt, col_steps=int(c1 - c0), attempts=attempts, ncols=ncols, sum_nz=int(nzs.sum()),
-                ni=cases[0].net.ni, factor_ms=(ms1 - ms0) / max(n1 - n0, 1))
+                ni=cases[0].net.ni, factor_ms=factor_ms)
 
 
 CPU_SNIPPET = r'''
@@ -141,6 +153,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--no-graph', action='store_true', help='eager launches instead of the captured HIP graph')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has one MI355X and the columns are '
@@ -152,7 +165,7 @@ def main():
         sys.exit(1)
     print('[bench] single column nz=150 ni=93: %d warmup + %d timed steps' % (args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    s = bench_single(args.steps, args.warmup)
+    s = bench_single(args.steps, args.warmup, graph=not args.no_graph)
     value = s['steps'] / s['time']
     nz, ni = s['nz'], s['ni']
     flops_launch = nz * (8.0 / 3.0) * ni ** 3
@@ -171,7 +184,8 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic',
         'config': {'workload': 'plain method, one column: nz=150, ni=93, 600 pairs (90 Troe) + 20 photolysis, '
-                               'dt0=1e-10 s', 'attempts': s['attempts'], 'model_time_s': s['t_model']},
+                               'dt0=1e-10 s', 'attempts': s['attempts'], 'model_time_s': s['t_model'],
+                   'launch': 'eager' if args.no_graph else 'hip-graph per attempt'},
         'roofline': {'kernel': 'k_factor (block-Thomas Gauss-Jordan inversion)', 'bound': 'mfma',
                      'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': achieved / FP64_PEAK, 'traffic': None,
@@ -179,7 +193,7 @@ def main():
     }
     if not args.no_batched:
         print('[bench] batched 256 columns: %d attempts' % args.batched_attempts, file=sys.stderr, flush=True)
-        bt = bench_batched(args.batched_attempts, 5)
+        bt = bench_batched(args.batched_attempts, 5, graph=not args.no_graph)
         fl = bt['sum_nz'] * (8.0 / 3.0) * bt['ni'] ** 3
         out['batched'] = {'workload': '256 columns, nz uniform [80,200], ni=93', 'value': bt['col_steps'] / bt['time'],
                           'unit': 'column-steps/s', 'attempts': bt['attempts'], 'col_steps': bt['col_steps'],

@@ -207,6 +207,14 @@ int vk_atom_sums(const vk_plan *plan, const vk_columns *cols, const double *d_y,
  * reports (summed milliseconds, number of timed launches). */
 int vk_integrator_set_timing(vk_integrator *it, int on);
 int vk_integrator_stats(const vk_integrator *it, double *factor_ms, long long *factor_launches);
+/* With graphs on (and timing off), vk_integrate captures one loop iteration
+ * (attempt, decision, bookkeeping, conv: about 25 kernels) into a HIP graph
+ * on a default-priority stream owned by the integrator and replays it once
+ * per attempt, ordered after and before the caller's stream with events.
+ * The graph is re-captured whenever the scalar arguments or device pointers
+ * change.  No counterpart in the reference (its loop is per-step Python);
+ * results are bitwise those of the eager launches. */
+int vk_integrator_set_graph(vk_integrator *it, int on);
 
 /* ---- warm-start MLP (no counterpart in the reference: it seeds the plain
  * Ros2 integration, which stays the fallback and the judge) --------------

@@ -326,3 +326,31 @@ def test_batch_is_column_independent():
         assert np.array_equal(b.column('y', i), bi.y.cpu().numpy())
         assert b.host_state()[i]['count'] == bi.host_state()[0]['count']
     del k_all
+
+
+@gpu
+def test_graph_replay_is_bitwise_eager():
+    """The captured per-attempt graph, replayed many times with host polls
+    and eager work in between, gives exactly the eager launches' results;
+    a parameter change forces a re-capture."""
+    torch = _torch()
+    cases = W.batched_cases(3, nz_range=(12, 30))
+    net, cfg = cases[0].net, cases[0].cfg
+    cfg.conv_step = 20
+    runs = []
+    for graph in (False, True):
+        b = _batch(net, [c.col for c in cases], [c.y0 for c in cases], cfg, history=cfg.conv_step + 1)
+        b.set_graph(graph)
+        for rep in range(6):
+            b.integrate(7)
+            b.host_state()                      # synchronising poll between replays
+            b.atom_loss_prev.add_(0.0)          # eager torch work on the same stream
+        b.integrate(5, use_update_mu=False)     # different scalar arguments: new capture
+        b.integrate(3)
+        torch.cuda.synchronize()
+        runs.append((b.y.cpu().numpy(), b.ymix.cpu().numpy(), b.y_prev.cpu().numpy(),
+                     b.host_state(), b.atom_loss.cpu().numpy()))
+    e, g = runs
+    for a_, b_ in zip(e, g):
+        assert np.array_equal(np.asarray(a_).view(np.uint8), np.asarray(b_).view(np.uint8))
+    assert e[3]['attempts'].min() == 6 * 7 + 8 or e[3]['done'].any()

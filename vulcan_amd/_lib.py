@@ -73,6 +73,7 @@ _SIGS = {
     'vk_update_mu_dz': [VP, VP, C.POINTER(Atm), C.POINTER(Geom), VP, VP, C.c_int, VP],
     'vk_atom_sums': [VP, VP, VP, VP, VP],
     'vk_integrator_set_timing': [VP, C.c_int],
+    'vk_integrator_set_graph': [VP, C.c_int],
     'vk_integrator_stats': [VP, c_double_p, C.POINTER(C.c_longlong)],
     'vk_mlp_gemm': [C.c_int, C.c_int, C.c_int, VP, C.c_long, C.c_long, VP, C.c_long, C.c_long, VP, C.c_long, VP,
                     C.c_int, VP, C.c_long, VP],

@@ -52,5 +52,9 @@ struct vk_integrator {
     int timing;                // time the factor kernel of the last attempt per call
     double factor_ms;          // summed duration of the timed factor launches
     long long factor_launches; // number of timed factor launches
+    int use_graph;             // replay one captured loop iteration per attempt
+    hipGraphExec_t gexec;
+    hipStream_t gstream;
+    std::vector<char> gkey;    // arguments baked into gexec
     std::vector<void *> allocs;
 };

@@ -534,6 +534,9 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
         return VK_ENOMEM;
     }
     it->timing = 0;
+    it->use_graph = 0;
+    it->gexec = nullptr;
+    it->gstream = nullptr;
     it->factor_ms = 0.0;
     it->factor_launches = 0;
     *out = it;
@@ -560,6 +563,8 @@ static int ensure_thist(vk_integrator *it, int count_max) {
 
 extern "C" int vk_integrator_destroy(vk_integrator *it) {
     if (!it) return 0;
+    if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
+    if (it->gstream) (void)hipStreamDestroy(it->gstream);
     for (void *q : it->allocs) (void)hipFree(q);
     if (it->thist) (void)hipFree(it->thist);
     if (it->ev0) (void)hipEventDestroy(it->ev0);
@@ -635,6 +640,35 @@ extern "C" int vk_ros2_solver(vk_integrator *it, const vk_atm *atm, const vk_par
     return 0;
 }
 
+// one full loop iteration (attempt + decision + bookkeeping) on stream s
+static int loop_body(vk_integrator *it, const vk_atm *atm, const vk_geom *g, const vk_params *prm,
+                     const double *d_k, double *d_y, double *d_ymix, double *d_y_prev, double *d_atom_ini,
+                     double *d_atom_loss, double *d_atom_loss_prev, vk_colstate *d_st, bool timed, hipStream_t s) {
+    const vk_plan *p = it->plan;
+    const vk_columns *cl = it->cols;
+    int r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, timed, s);
+    if (r) return r;
+    hipLaunchKernelGGL(k_decide, dim3(cl->ncols), dim3(1), 0, s, p->natom, cl->d_off, it->part, *prm, 1,
+                       d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->flags, it->thist, it->tcap);
+    if (prm->use_update_mu)
+        hipLaunchKernelGGL(k_update_mu, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, p->ms, d_ymix, *atm,
+                           *g, it->flags, 0, d_st);
+    hipLaunchKernelGGL(k_apply, dim3(cl->L), dim3(128), 0, s, p->ni, cl->d_layer_col, it->flags, atm->M, d_ymix, d_y,
+                       d_y_prev, d_st, it->yhist, it->history, (size_t)cl->L * p->ni);
+    hipLaunchKernelGGL(k_conv, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, *prm, *atm, d_y, d_ymix,
+                       it->flags, d_st, it->yhist, it->history, (size_t)cl->L * p->ni, it->thist, it->tcap);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+// everything the captured graph bakes in
+struct GraphKey {
+    vk_atm atm;
+    vk_geom geom;
+    vk_params prm;
+    const void *ptr[9];
+};
+
 extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom, const vk_params *prm,
                             const double *d_k, double *d_y, double *d_ymix, double *d_y_prev, double *d_atom_ini,
                             double *d_atom_loss, double *d_atom_loss_prev, vk_colstate *d_st, int n_attempts,
@@ -645,27 +679,54 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     if (prm->use_update_mu && !geom) return VK_EINVAL;
     if (prm->use_conv && it->history > 0 && it->history < prm->conv_step + 1) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
-    const vk_plan *p = it->plan;
-    const vk_columns *cl = it->cols;
     int r = ensure_thist(it, prm->count_max);
     if (r) return r;
     vk_geom g{};
     if (geom) g = *geom;
-    for (int a = 0; a < n_attempts; a++) {
-        r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, it->timing && a == n_attempts - 1, s);
-        if (r) return r;
-        hipLaunchKernelGGL(k_decide, dim3(cl->ncols), dim3(1), 0, s, p->natom, cl->d_off, it->part, *prm, 1,
-                           d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->flags, it->thist, it->tcap);
-        if (prm->use_update_mu)
-            hipLaunchKernelGGL(k_update_mu, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, p->ms, d_ymix,
-                               *atm, g, it->flags, 0, d_st);
-        hipLaunchKernelGGL(k_apply, dim3(cl->L), dim3(128), 0, s, p->ni, cl->d_layer_col, it->flags, atm->M, d_ymix,
-                           d_y, d_y_prev, d_st, it->yhist, it->history, (size_t)cl->L * p->ni);
-        hipLaunchKernelGGL(k_conv, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, *prm, *atm, d_y, d_ymix,
-                           it->flags, d_st, it->yhist, it->history, (size_t)cl->L * p->ni, it->thist, it->tcap);
-        VK_CHECK(hipGetLastError());
+    if (!it->use_graph || it->timing) {
+        for (int a = 0; a < n_attempts; a++) {
+            r = loop_body(it, atm, &g, prm, d_k, d_y, d_ymix, d_y_prev, d_atom_ini, d_atom_loss, d_atom_loss_prev,
+                          d_st, it->timing && a == n_attempts - 1, s);
+            if (r) return r;
+        }
+        if (n_attempts > 0 && it->timing) accumulate_factor_time(it);
+        return 0;
     }
-    if (n_attempts > 0 && it->timing) accumulate_factor_time(it);
+    // graph path: one captured loop iteration replayed per attempt on the
+    // integrator's own stream, ordered after and before the caller's stream
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.atm = *atm;
+    key.geom = g;
+    key.prm = *prm;
+    const void *ptrs[9] = {d_k, d_y, d_ymix, d_y_prev, d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->thist};
+    memcpy(key.ptr, ptrs, sizeof(ptrs));
+    if (!it->gstream) VK_CHECK(hipStreamCreateWithFlags(&it->gstream, hipStreamNonBlocking));
+    if (!it->gexec || it->gkey.size() != sizeof(key) || memcmp(it->gkey.data(), &key, sizeof(key)) != 0) {
+        if (it->gexec) { (void)hipGraphExecDestroy(it->gexec); it->gexec = nullptr; }
+        hipGraph_t graph = nullptr;
+        VK_CHECK(hipStreamBeginCapture(it->gstream, hipStreamCaptureModeThreadLocal));
+        r = loop_body(it, atm, &g, prm, d_k, d_y, d_ymix, d_y_prev, d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st,
+                      false, it->gstream);
+        hipError_t e = hipStreamEndCapture(it->gstream, &graph);
+        if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        VK_CHECK(e);
+        e = hipGraphInstantiate(&it->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        VK_CHECK(e);
+        it->gkey.assign((const char *)&key, (const char *)&key + sizeof(key));
+    }
+    VK_CHECK(hipEventRecord(it->ev0, s));
+    VK_CHECK(hipStreamWaitEvent(it->gstream, it->ev0, 0));
+    for (int a = 0; a < n_attempts; a++) VK_CHECK(hipGraphLaunch(it->gexec, it->gstream));
+    VK_CHECK(hipEventRecord(it->ev1, it->gstream));
+    VK_CHECK(hipStreamWaitEvent(s, it->ev1, 0));
+    return 0;
+}
+
+extern "C" int vk_integrator_set_graph(vk_integrator *it, int on) {
+    if (!it) return VK_EINVAL;
+    it->use_graph = on ? 1 : 0;
     return 0;
 }
 

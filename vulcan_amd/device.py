@@ -147,7 +147,7 @@ class ColumnBatch:
     """
 
     def __init__(self, plan: DevicePlan, cols: List[Column], y0: Sequence[np.ndarray], cfg: Config,
-                 history: int = 0, k: Optional[np.ndarray] = None):
+                 history: int = 0, k: Optional[np.ndarray] = None, graph: bool = False):
         cfg.check_supported()
         self.plan, self.cfg = plan, cfg
         net = plan.net
@@ -204,6 +204,7 @@ class ColumnBatch:
         _lib.check(lib.vk_integrator_create(plan.handle, self.layout.handle, history, C.byref(h)),
                    'vk_integrator_create')
         self.it = h
+        self.set_graph(graph)
         self.atm = _lib.Atm(Tco=dptr(self.Tco), M=dptr(self.M), Kzz=dptr(self.Kzz), Dzz=dptr(self.Dzz),
                             dzi=dptr(self.dzi), Ti=dptr(self.Ti), Hpi=dptr(self.Hpi), g=dptr(self.g),
                             Hp=dptr(self.Hp), tau=dptr(self.tau))
@@ -245,6 +246,10 @@ class ColumnBatch:
                                           dptr(self.k), dptr(self.y), dptr(self.ymix), dptr(self.y_prev),
                                           dptr(self.atom_ini), dptr(self.atom_loss), dptr(self.atom_loss_prev),
                                           dptr(self.state), int(n_attempts), stream_handle()), 'vk_integrate')
+
+    def set_graph(self, on: bool):
+        """Replay a captured HIP graph per attempt in ``integrate``."""
+        _lib.check(self._lib.vk_integrator_set_graph(self.it, int(on)), 'vk_integrator_set_graph')
 
     def set_timing(self, on: bool):
         _lib.check(self._lib.vk_integrator_set_timing(self.it, int(on)), 'vk_integrator_set_timing')
