@@ -161,6 +161,31 @@ int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
 int vk_block_factor(const vk_plan *plan, const vk_columns *cols, const double *d_D,
                     const double *d_Lsub, const double *d_Usup, double *d_Dinv,
                     int *d_info, void *stream);
+/* Block-inversion kernel of the factorisation (no counterpart in the
+ * reference, whose banded LU has one code path): VK_FACTOR_REGISTER keeps
+ * the block in registers and runs one barrier-synchronised Gauss-Jordan
+ * step per pivot; VK_FACTOR_BLOCKED groups the pivots in panels of 16,
+ * factors each panel in one wave and applies the rank-16 Schur update on
+ * FP64 MFMA (compiled for one or for two resident workgroups per CU).  All
+ * give the inverse to rounding; vk_tune_factor times every
+ * variant on a synthetic system of the columns' shape (`reps` launches
+ * each, milliseconds per launch into ms[VK_FACTOR_NVARIANTS]) and keeps the
+ * fastest in the plan. */
+#define VK_FACTOR_REGISTER     0
+#define VK_FACTOR_BLOCKED      1   /* register budget for one workgroup per CU      */
+#define VK_FACTOR_BLOCKED_OCC2 2   /* register budget for two workgroups per CU     */
+#define VK_FACTOR_NVARIANTS    3
+int vk_plan_set_factor_variant(vk_plan *plan, int variant);
+int vk_plan_factor_variant(const vk_plan *plan);
+int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, float *ms, void *stream);
+/* Self-test of the FP64 MFMA operand layout the blocked inversion relies
+ * on: out[i*16+j] = sum_k A[i][k] B[k][j] with A[i][k] = i + k/4 + 1,
+ * B[k][j] = 1 - k/2 + j/8 (16 x 4 x 16), computed by one MFMA. */
+int vk_selftest_mfma_f64(double *d_out, void *stream);
+/* Self-test of the wave-wide maximum used by the blocked inversion's pivot
+ * search: out[l] = max over the 64 lanes of in[], for every lane l. */
+int vk_selftest_wave_max_u64(const unsigned long long *d_in, unsigned long long *d_out, void *stream);
+
 /* replaces the gbtrs half of solve_banded (op.py:2914, 2929): rhs <- lhs^-1 rhs */
 int vk_block_solve(const vk_plan *plan, const vk_columns *cols, const double *d_Dinv,
                    const double *d_Lsub, const double *d_Usup, double *d_rhs, void *stream);

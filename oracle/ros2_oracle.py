@@ -431,8 +431,28 @@ def new_state(net, y, k, cfg: Config) -> State:
     return st
 
 
-def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False):
-    """One Ros2 attempt (op.py:2860-3007).  Mutates st.y, st.ymix, st.delta."""
+def solve_refined(D, L, U, lhs_b, bw, rhs, iters=3):
+    """The banded LAPACK solve followed by `iters` steps of iterative
+    refinement with the residual b - A x formed blockwise in extended
+    precision: the linear solve to (nearly) full FP64 accuracy.  Not part of
+    the reference; it gives the exact Ros2 step against which the
+    reference's own banded solve and the device solve are both measured."""
+    nz, ni = D.shape[:2]
+    Dl, Ll, Ul = D.astype(np.longdouble), L.astype(np.longdouble), U.astype(np.longdouble)
+    bl = rhs.reshape(nz, ni).astype(np.longdouble)
+    x = scipy.linalg.solve_banded((bw, bw), lhs_b, rhs).reshape(nz, ni).astype(np.longdouble)
+    for _ in range(iters):
+        r = bl - np.einsum('lij,lj->li', Dl, x)
+        r[1:] -= Ll[1:] * x[:-1]
+        r[:-1] -= Ul[:-1] * x[1:]
+        dx = scipy.linalg.solve_banded((bw, bw), lhs_b, r.astype(np.float64).ravel())
+        x = x + dx.reshape(nz, ni).astype(np.longdouble)
+    return x.astype(np.float64).ravel()
+
+
+def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False, exact=False):
+    """One Ros2 attempt (op.py:2860-3007).  Mutates st.y, st.ymix, st.delta.
+    exact=True replaces the two banded solves by solve_refined."""
     tables = tables or nw.build_tables(net)
     y, ymix, h, k = st.y, st.ymix, st.dt, st.k
     M = col.M
@@ -458,12 +478,14 @@ def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False):
         del lhs
     else:
         lhs_b, bw = band_from_blocks(D, L, U)
-    k1_flat = scipy.linalg.solve_banded((bw, bw), lhs_b, df)
+    solve = ((lambda rhs_: solve_refined(D, L, U, lhs_b, bw, rhs_)) if exact
+             else (lambda rhs_: scipy.linalg.solve_banded((bw, bw), lhs_b, rhs_)))
+    k1_flat = solve(df)
     k1 = k1_flat.reshape(y.shape)
     yk2 = y + k1 / r
     df = chemdf(net, yk2, M, k, tables).flatten() + diffdf(col, yk2, cfg.use_moldiff).flatten()
     rhs = df - 2. / (r * h) * k1_flat
-    k2 = scipy.linalg.solve_banded((bw, bw), lhs_b, rhs).reshape(y.shape)
+    k2 = solve(rhs).reshape(y.shape)
     sol = y + 3. / (2. * r) * k1 + 1 / (2. * r) * k2
     delta = np.abs(sol - yk2)
     delta[ymix < cfg.mtol] = 0
@@ -514,11 +536,11 @@ def step_reject(st: State, cfg: Config) -> bool:
     return False
 
 
-def one_step(net, col, st: State, cfg: Config, tables=None, max_attempts=10 ** 6, dense_lhs=False):
+def one_step(net, col, st: State, cfg: Config, tables=None, max_attempts=10 ** 6, dense_lhs=False, exact=False):
     """op.py:3091-3103."""
     atom_mat = net.atom_matrix()
     for _ in range(max_attempts):
-        ros2_solver(net, col, st, cfg, tables, dense_lhs=dense_lhs)
+        ros2_solver(net, col, st, cfg, tables, dense_lhs=dense_lhs, exact=exact)
         clip(st, atom_mat, cfg)
         if step_ok(st, cfg):
             break

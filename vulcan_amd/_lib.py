@@ -63,6 +63,11 @@ _SIGS = {
     'vk_jac': [VP, VP, VP, VP, VP, VP, VP],
     'vk_lhs': [VP, VP, C.POINTER(Atm), VP, VP, C.c_int, VP, VP, VP, VP, VP],
     'vk_block_factor': [VP, VP, VP, VP, VP, VP, VP, VP],
+    'vk_plan_set_factor_variant': [VP, C.c_int],
+    'vk_plan_factor_variant': [VP],
+    'vk_tune_factor': [VP, VP, C.c_int, C.POINTER(C.c_float), VP],
+    'vk_selftest_mfma_f64': [VP, VP],
+    'vk_selftest_wave_max_u64': [VP, VP, VP],
     'vk_block_solve': [VP, VP, VP, VP, VP, VP, VP],
     'vk_block_residual': [VP, VP, VP, VP, VP, VP, VP, VP, VP],
     'vk_integrator_create': [VP, VP, C.c_int, C.POINTER(VP)],

@@ -228,15 +228,16 @@ __global__ void __launch_bounds__(BLK) k_rhs(
 __global__ void __launch_bounds__(BLK) k_jac_lhs(
     int ni, int npairs, int nslot, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
-    const int *__restrict__ slot_sp, const int *__restrict__ jac_map, const int *__restrict__ jac_ptr,
-    const int *__restrict__ jac_slot, const double *__restrict__ jac_coef, const double *__restrict__ ms,
-    const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
-    const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
-    const vk_colstate *__restrict__ st, double *__restrict__ D, double *__restrict__ Lsub,
-    double *__restrict__ Usup) {
+    const int *__restrict__ slot_sp, const int2 *__restrict__ jac_rng, const int *__restrict__ jac_slot,
+    const double *__restrict__ jac_coef, const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm,
+    const double *__restrict__ y, const double *__restrict__ k, const double *__restrict__ M, int mode, int mol,
+    const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ D,
+    double *__restrict__ Lsub, double *__restrict__ Usup) {
     extern __shared__ double smem[];
-    double *yl = smem;               // ni
-    double *dv = smem + VK_NI_MAX;   // nslot
+    double *yl = smem;                       // ni
+    double *dg0 = smem + VK_NI_MAX;          // diffusion terms of the diagonal, per species
+    double *dg1 = smem + 2 * VK_NI_MAX;
+    double *dv = smem + 3 * VK_NI_MAX;       // nslot
     __shared__ double ysum3[3];
     const int l = blockIdx.x;
     LayerCtx x = layer_ctx(off, layer_col, l);
@@ -259,43 +260,44 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         double dp = dside(prod + 4 * p, kl[2 * p + 1], yl, m, sp, a2);
         dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
     }
-    __syncthreads();
-    const int nn = ni * ni;
-    double *Dl = D + (size_t)l * nn;
     double c0 = 0.0;
-    if (mode == 1) c0 = 1. / (R_ROS * dtc[x.c]);
-    ColAtm ca = col_atm(atm, x.off, x.nz, ni);
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-        int jm = jac_map[e];
-        double J = 0.0;
-        if (jm >= 0) {
-            int lo = jac_ptr[jm], hi = jac_ptr[jm + 1];
-            J = jac_coef[lo] * dv[jac_slot[lo]];
-            for (int q = lo + 1; q < hi; q++) J = J + jac_coef[q] * dv[jac_slot[q]];
-        }
-        if (mode == 0) {
-            Dl[e] = J;
-            continue;
-        }
-        int i = e / ni, mm = e - i * ni;
-        double val = -J;
-        if (i == mm) {
-            val = c0 + val;
-            LhsDiff r = lhs_diff(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ms[i], alpha[i], mol != 0);
-            val = val - r.d0;
-            if (r.has_mol) val = val - r.d1;
-        }
-        Dl[e] = val;
-    }
     if (mode == 1) {
+        // diffusion part of the lhs (op.py:2012-2040): once per species
+        c0 = 1. / (R_ROS * dtc[x.c]);
+        ColAtm ca = col_atm(atm, x.off, x.nz, ni);
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             LhsDiff r = lhs_diff(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ms[i], alpha[i], mol != 0);
+            dg0[i] = r.d0;
+            dg1[i] = r.d1;
             double u = 0.0, lo = 0.0;
             if (r.has_u) { u = 0.0 - r.u0; if (r.has_mol) u = u - r.u1; }
             if (r.has_l) { lo = 0.0 - r.l0; if (r.has_mol) lo = lo - r.l1; }
             Usup[(size_t)l * ni + i] = u;
             Lsub[(size_t)l * ni + i] = lo;
         }
+    }
+    __syncthreads();
+    const int nn = ni * ni;
+    double *Dl = D + (size_t)l * nn;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        const int2 rg = jac_rng[e];
+        double J = 0.0;
+        if (rg.y > rg.x) {
+            J = jac_coef[rg.x] * dv[jac_slot[rg.x]];
+            for (int q = rg.x + 1; q < rg.y; q++) J = J + jac_coef[q] * dv[jac_slot[q]];
+        }
+        if (mode == 0) {
+            Dl[e] = J;
+            continue;
+        }
+        const int i = e / ni, mm = e - i * ni;
+        double val = -J;
+        if (i == mm) {
+            val = c0 + val;
+            val = val - dg0[i];
+            if (mol) val = val - dg1[i];
+        }
+        Dl[e] = val;
     }
 }
 
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
 static size_t rhs_smem(const vk_plan *p) { return sizeof(double) * (VK_NI_MAX + (size_t)p->npairs); }
-static size_t jac_smem(const vk_plan *p) { return sizeof(double) * (VK_NI_MAX + (size_t)p->nslot); }
+static size_t jac_smem(const vk_plan *p) { return sizeof(double) * (3 * VK_NI_MAX + (size_t)p->nslot); }
 
 extern "C" int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, double *d_k, void *stream) {
     if (!plan || !cols || !atm || !d_k) return VK_EINVAL;
@@ -343,8 +345,8 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     if (atm) a = *atm;
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair, plan->slot_sp,
-                       plan->jac_map, plan->jac_ptr, plan->jac_slot, plan->jac_coef, plan->ms, plan->alpha, a, y, k,
-                       M, mode, mol, dtc, st, D, Lsub, Usup);
+                       plan->jac_rng, plan->jac_slot, plan->jac_coef, plan->ms, plan->alpha, a, y, k, M, mode, mol,
+                       dtc, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;
 }

@@ -13,13 +13,16 @@
         if (e_ != hipSuccess) return -(int)e_;                           \
     } while (0)
 
+
 struct vk_plan {
     int ni, npairs, nr, natom, nchem, nslot, njnz, njac;
+    int factor_variant;
     int *reac, *prod, *kind, *rev;
     double *par, *nasa9, *atom_mat, *ms, *alpha;
     int *chem_ptr, *chem_pair, *chem_sign;
     int *slot_pair, *slot_sp;
     int *jac_map, *jac_ptr, *jac_slot;
+    int2 *jac_rng;             // per dense entry: [jac_ptr[jm], jac_ptr[jm+1]) or empty
     double *jac_coef;
     std::vector<void *> allocs;
 };

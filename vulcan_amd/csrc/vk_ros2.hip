@@ -415,6 +415,7 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
     if (n->ni > VK_NI_MAX || n->natom > VK_NATOM_MAX) return VK_ETOOBIG;
     vk_plan *p = new vk_plan();
     p->ni = n->ni; p->npairs = n->npairs; p->nr = 2 * n->npairs; p->natom = n->natom;
+    p->factor_variant = VK_FACTOR_REGISTER;
     p->nchem = n->nchem; p->nslot = n->nslot; p->njnz = n->njnz; p->njac = n->njac;
     int r = 0;
     auto &v = p->allocs;
@@ -436,6 +437,14 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
     r = r ? r : upload(v, &p->jac_ptr, n->jac_ptr, (size_t)n->njnz + 1);
     r = r ? r : upload(v, &p->jac_slot, n->jac_slot, (size_t)n->njac);
     r = r ? r : upload(v, &p->jac_coef, n->jac_coef, (size_t)n->njac);
+    if (!r) {
+        std::vector<int2> rng((size_t)n->ni * n->ni);
+        for (size_t e = 0; e < rng.size(); e++) {
+            const int jm = n->jac_map[e];
+            rng[e] = jm >= 0 ? make_int2(n->jac_ptr[jm], n->jac_ptr[jm + 1]) : make_int2(0, 0);
+        }
+        r = upload(v, &p->jac_rng, rng.data(), rng.size());
+    }
     if (r) { vk_plan_destroy(p); return r; }
     *out = p;
     return 0;
@@ -667,6 +676,7 @@ struct GraphKey {
     vk_geom geom;
     vk_params prm;
     const void *ptr[9];
+    int factor_variant;
 };
 
 extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom, const vk_params *prm,
@@ -701,6 +711,7 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     key.prm = *prm;
     const void *ptrs[9] = {d_k, d_y, d_ymix, d_y_prev, d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->thist};
     memcpy(key.ptr, ptrs, sizeof(ptrs));
+    key.factor_variant = it->plan->factor_variant;
     if (!it->gstream) VK_CHECK(hipStreamCreateWithFlags(&it->gstream, hipStreamNonBlocking));
     if (!it->gexec || it->gkey.size() != sizeof(key) || memcmp(it->gkey.data(), &key, sizeof(key)) != 0) {
         if (it->gexec) { (void)hipGraphExecDestroy(it->gexec); it->gexec = nullptr; }

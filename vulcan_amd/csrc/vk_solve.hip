@@ -207,6 +207,355 @@ __device__ __forceinline__ void load_schur(double (&a)[RPT][CPT], int ni, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inversion on FP64 MFMA (factor variant 1)
+//
+// The pivot steps are grouped into panels of 16 columns.  One wave runs the
+// 16 pivot steps of a panel on the panel columns alone (rows spread over the
+// lanes, pivot search by a wave-wide max over (|value|, row) keys, pivot row
+// broadcast with readlane: no workgroup barrier inside a panel).  The other
+// columns then receive the whole panel at once, as the rank-16 Schur update
+//     A[i][j] = (i in P ? 0 : A[i][j]) + sum_s Pan[i][s] A_old[p_s][j]
+// (P: the panel's pivot rows p_s, Pan: the factored panel columns), which is
+// a 16-deep GEMM and runs on v_mfma_f64_16x16x4f64.  Three barriers per
+// panel instead of one per pivot step.  The matrix lives in the MFMA
+// accumulator layout: 16 x 16 tiles, wave (wr, wc) of the 2 x 2 wave grid
+// owns row blocks wr, wr+2, ... and column blocks wc, wc+2, ...; lane l,
+// register r of a tile holds element (4*r + l/16, l%16) (operands: A(i, k)
+// in lane i + 16 k, B(k, j) in lane j + 16 k; checked by
+// vk_selftest_mfma_f64).
+// Pivoting is implicit (rows are not swapped): ipos[r] is the pivot step at
+// which row r was chosen, perm[k] the pivot row of step k, and the inverse is
+// inv[ipos[r]][perm[c]] = A[r][c] at the end.
+// ---------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+struct BlkShared {
+    static constexpr int N16 = 16 * NB;
+    double pan[2][16][N16 + 1];   // panel columns, [k][row], double-buffered by panel parity
+    double rb[16][N16];           // old pivot rows of the panel, [s][col]
+    int ipos[N16];
+    int perm[N16];
+    int info;
+};
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffull), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xf, 0xf, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int lane) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Maximum over the 64 lanes of a wave (all lanes active), returned
+// wave-uniform: DPP within rows of 16 (quad swaps, half-row and row
+// mirrors), then the four row maxima through readlane.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    unsigned long long u;
+    u = dpp_u64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = u > v ? u : v;
+    u = dpp_u64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = u > v ? u : v;
+    u = dpp_u64<0x141>(v);   // row_half_mirror
+    v = u > v ? u : v;
+    u = dpp_u64<0x140>(v);   // row_mirror
+    v = u > v ? u : v;
+    unsigned long long m = readlane_u64(v, 0);
+    u = readlane_u64(v, 16);
+    m = u > m ? u : m;
+    u = readlane_u64(v, 32);
+    m = u > m ? u : m;
+    u = readlane_u64(v, 48);
+    return u > m ? u : m;
+}
+
+// Pivot steps p0 .. p0+kend-1 on the panel pan[k][row] (one wave).
+template <int NB>
+__device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * NB + 1], int ni, int p0, int kend,
+                                          int tag) {
+    constexpr int N16 = 16 * NB;
+    constexpr int RPL = (N16 + 63) / 64;
+    const int l = threadIdx.x & 63;
+    double v[RPL][16];
+    bool av[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const int r = l + 64 * q;
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[q][k] = (r < N16) ? pan[k][r] : 0.0;
+        av[q] = (r < ni) && sh.ipos[r] < 0;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+        if (kk >= kend) continue;   // uniform; keeps the trip count static
+        unsigned long long best = 0;
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            const int r = l + 64 * q;
+            if (av[q]) {
+                const unsigned long long key =
+                    ((unsigned long long)__double_as_longlong(fabs(v[q][kk])) & ~0xffull) | (unsigned)(255 - r);
+                best = key > best ? key : best;
+            }
+        }
+        best = wave_max_u64(best);
+        const unsigned blo = (unsigned)(best & 0xffffffffull);
+        const unsigned bhi = (unsigned)(best >> 32);
+        const int p = (blo | bhi) ? 255 - (int)(blo & 0xffu) : 0;
+        if (((blo >> 8) | bhi) == 0u && l == 0 && sh.info == 0) sh.info = tag;   // zero (or no) pivot
+        const int pl = p & 63, pq = p >> 6;
+        double rowv[16];
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (q == pq) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) rowv[k] = readlane_f64(v[q][k], pl);
+            }
+        const double inv = 1.0 / rowv[kk];
+        double rs[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) rs[k] = rowv[k] * inv;
+#pragma unroll
+        for (int q = 0; q < RPL; q++) {
+            const bool isp = (l + 64 * q == p);
+            const double f = v[q][kk];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const double u = fma(-f, rs[k], v[q][k]);
+                v[q][k] = isp ? rs[k] : u;
+            }
+            v[q][kk] = isp ? inv : -f * inv;
+            av[q] = av[q] && !isp;
+        }
+        if (l == pl) sh.ipos[p] = p0 + kk;
+        if (l == 0) sh.perm[p0 + kk] = p;
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const int r = l + 64 * q;
+        if (r < N16) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) pan[k][r] = v[q][k];
+        }
+    }
+    for (int s = kend; s < 16; s++)
+        for (int c = l; c < N16; c += 64) sh.rb[s][c] = 0.0;
+}
+
+// Inverts the matrix held in acc (MFMA tile layout) and writes the inverse
+// transposed to X (X[m][i] = inv[i][m]).  A zero pivot records `tag` in
+// sh.info (sticky).
+template <int NB>
+__device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni, double *__restrict__ X,
+                                           BlkShared<NB> &sh, int tag) {
+    constexpr int N16 = 16 * NB, TB = (NB + 1) / 2;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int wr = w >> 1, wc = w & 1;
+    const int lg = l >> 4, lc = l & 15;
+    for (int t = tid; t < N16; t += NT) sh.ipos[t] = -1;
+    __syncthreads();
+    const int npan = (ni + 15) >> 4;
+    for (int pb = 0; pb < npan; pb++) {
+        double(*pan)[N16 + 1] = sh.pan[pb & 1];
+        const int p0 = 16 * pb, kend = min(16, ni - p0);
+        // the owners of column block pb publish the panel
+#pragma unroll
+        for (int tj = 0; tj < TB; tj++) {
+            if (wc + 2 * tj != pb) continue;   // static register index, uniform test
+#pragma unroll
+            for (int ti = 0; ti < TB; ti++) {
+                const int bi = wr + 2 * ti;
+                if (bi < NB) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) pan[lc][16 * bi + 4 * r + lg] = acc[ti][tj][r];
+                }
+            }
+        }
+        __syncthreads();
+        if (w == 0) blk_panel<NB>(sh, pan, ni, p0, kend, tag);
+        __syncthreads();
+        // panel tiles take the factored panel; the other tiles hand their
+        // pivot-row entries to rb and clear them
+#pragma unroll
+        for (int ti = 0; ti < TB; ti++) {
+            const int bi = wr + 2 * ti;
+            if (bi >= NB) continue;
+            int sr[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) sr[r] = sh.ipos[16 * bi + 4 * r + lg] - p0;
+#pragma unroll
+            for (int tj = 0; tj < TB; tj++) {
+                const int bj = wc + 2 * tj;
+                if (bj >= NB) continue;
+                if (bj == pb) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[ti][tj][r] = pan[lc][16 * bi + 4 * r + lg];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (sr[r] >= 0 && sr[r] < 16) {
+                            sh.rb[sr[r]][16 * bj + lc] = acc[ti][tj][r];
+                            acc[ti][tj][r] = 0.0;
+                        }
+                }
+            }
+        }
+        __syncthreads();
+        // rank-16 update on MFMA
+#pragma unroll
+        for (int ti = 0; ti < TB; ti++) {
+            const int bi = wr + 2 * ti;
+            if (bi >= NB) continue;
+            double a[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) a[s4] = pan[4 * s4 + (l >> 4)][16 * bi + lc];
+#pragma unroll
+            for (int tj = 0; tj < TB; tj++) {
+                const int bj = wc + 2 * tj;
+                if (bj >= NB || bj == pb) continue;
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], sh.rb[4 * s4 + (l >> 4)][16 * bj + lc],
+                                                                        acc[ti][tj], 0, 0, 0);
+            }
+        }
+    }
+    // inv[ipos[r]][perm[c]] = A[r][c]  ->  X[perm[c]][ipos[r]]
+#pragma unroll
+    for (int ti = 0; ti < TB; ti++) {
+        const int bi = wr + 2 * ti;
+        if (bi >= NB) continue;
+        int qi[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 16 * bi + 4 * r + lg;
+            qi[r] = i < ni ? sh.ipos[i] : -1;
+        }
+#pragma unroll
+        for (int tj = 0; tj < TB; tj++) {
+            const int bj = wc + 2 * tj;
+            const int cc = 16 * bj + lc;
+            if (bj >= NB || cc >= ni) continue;
+            const size_t rowoff = (size_t)sh.perm[cc] * ni;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (qi[r] >= 0) X[rowoff + qi[r]] = acc[ti][tj][r];
+        }
+    }
+    __syncthreads();
+}
+
+// acc = D_l - diag(cl) Xn^T diag(cn) [- diag(cl2) Xn2^T diag(cn2)] in the tile layout
+template <int NB>
+__device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni,
+                                               const double *__restrict__ Dl, const double *cl, const double *Xn,
+                                               const double *cn, const double *cl2, const double *Xn2,
+                                               const double *cn2) {
+    constexpr int TB = (NB + 1) / 2;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int wr = w >> 1, wc = w & 1;
+    const int lg = l >> 4, lc = l & 15;
+#pragma unroll
+    for (int ti = 0; ti < TB; ti++) {
+        const int bi = wr + 2 * ti;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 16 * bi + 4 * r + lg;
+            const bool rowok = bi < NB && i < ni;
+            const double li = (Xn && rowok) ? cl[i] : 0.0;
+            const double li2 = (Xn2 && rowok) ? cl2[i] : 0.0;
+#pragma unroll
+            for (int tj = 0; tj < TB; tj++) {
+                const int bj = wc + 2 * tj;
+                const int cc = 16 * bj + lc;
+                double v = 0.0;
+                if (rowok && bj < NB && cc < ni) {
+                    v = Dl[(size_t)i * ni + cc];
+                    if (Xn) v = v - li * (Xn[(size_t)cc * ni + i] * cn[cc]);
+                    if (Xn2) v = v - li2 * (Xn2[(size_t)cc * ni + i] * cn2[cc]);
+                }
+                acc[ti][tj][r] = v;
+            }
+        }
+    }
+}
+
+template <int NB, int MINW>
+__global__ void __launch_bounds__(NT, MINW) k_factor_blk(int ni, const int *__restrict__ off, const double *__restrict__ D,
+                                                    const double *__restrict__ Lsub, const double *__restrict__ Usup,
+                                                    double *__restrict__ Dinv, int *__restrict__ info,
+                                                    const vk_colstate *__restrict__ st) {
+    const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
+    if (st && st[c].done) return;
+    __shared__ BlkShared<NB> sh;
+    if (threadIdx.x == 0) sh.info = 0;
+    const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
+    const size_t nn = (size_t)ni * ni;
+    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    if (side == 0) {
+        for (int j = 0; j < m; ++j) {
+            const size_t l = (size_t)l0 + j;
+            blk_load_schur<NB>(acc, ni, D + l * nn, Lsub + l * ni, j > 0 ? Dinv + (l - 1) * nn : nullptr,
+                               j > 0 ? Usup + (l - 1) * ni : nullptr, nullptr, nullptr, nullptr);
+            blk_invert<NB>(acc, ni, Dinv + l * nn, sh, j + 1);
+        }
+    } else {
+        for (int j = nz - 1; j > m; --j) {
+            const size_t l = (size_t)l0 + j;
+            blk_load_schur<NB>(acc, ni, D + l * nn, Usup + l * ni, j < nz - 1 ? Dinv + (l + 1) * nn : nullptr,
+                               j < nz - 1 ? Lsub + (l + 1) * ni : nullptr, nullptr, nullptr, nullptr);
+            blk_invert<NB>(acc, ni, Dinv + l * nn, sh, j + 1);
+        }
+    }
+    if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
+}
+
+template <int NB, int MINW>
+__global__ void __launch_bounds__(NT, MINW) k_factor_blk_mid(int ni, const int *__restrict__ off,
+                                                        const double *__restrict__ D, const double *__restrict__ Lsub,
+                                                        const double *__restrict__ Usup, double *__restrict__ Dinv,
+                                                        int *__restrict__ info, const vk_colstate *__restrict__ st) {
+    const int c = blockIdx.x;
+    if (st && st[c].done) return;
+    __shared__ BlkShared<NB> sh;
+    if (threadIdx.x == 0) sh.info = 0;
+    const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
+    const size_t nn = (size_t)ni * ni;
+    const size_t l = (size_t)l0 + m;
+    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    blk_load_schur<NB>(acc, ni, D + l * nn, Lsub + l * ni, m > 0 ? Dinv + (l - 1) * nn : nullptr,
+                       m > 0 ? Usup + (l - 1) * ni : nullptr, Usup + l * ni,
+                       m < nz - 1 ? Dinv + (l + 1) * nn : nullptr, m < nz - 1 ? Lsub + (l + 1) * ni : nullptr);
+    blk_invert<NB>(acc, ni, Dinv + l * nn, sh, m + 1);
+    if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
+}
+
+// D = A B for A[i][k] = i + 0.25 k + 1, B[k][j] = 1 - 0.5 k + 0.125 j (16 x 4 x 16)
+// through one v_mfma_f64_16x16x4f64 with the operand layout the blocked
+// inversion assumes; out[i*16+j] = D[i][j].
+__global__ void k_selftest_mfma_f64(double *out) {
+    const int l = threadIdx.x & 63;
+    const double a = (double)(l & 15) + 0.25 * (double)(l >> 4) + 1.0;
+    const double b = 1.0 - 0.5 * (double)(l >> 4) + 0.125 * (double)(l & 15);
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[(4 * r + (l >> 4)) * 16 + (l & 15)] = acc[r];
+}
+
 // Twisted block-Thomas: the lower chain (side 0) runs j = 0 .. m-1 with
 //   S_j = D_j - diag(Lsub_j) S_{j-1}^-1 diag(Usup_{j-1}),
 // the upper chain (side 1) runs j = nz-1 .. m+1 with
@@ -262,9 +611,9 @@ __global__ void __launch_bounds__(NT) k_factor_mid(int ni, const int *__restrict
     const size_t nn = (size_t)ni * ni;
     const size_t l = (size_t)l0 + m;
     double a[RPT][CPT];
-    load_schur<RPT, CPT>(a, ni, D + l * nn, Lsub + l * ni, Dinv + (l - 1) * nn, Usup + (l - 1) * ni,
-                         Usup + l * ni, m < nz - 1 ? Dinv + (l + 1) * nn : nullptr,
-                         m < nz - 1 ? Lsub + (l + 1) * ni : nullptr);
+    load_schur<RPT, CPT>(a, ni, D + l * nn, Lsub + l * ni, m > 0 ? Dinv + (l - 1) * nn : nullptr,
+                         m > 0 ? Usup + (l - 1) * ni : nullptr, Usup + l * ni,
+                         m < nz - 1 ? Dinv + (l + 1) * nn : nullptr, m < nz - 1 ? Lsub + (l + 1) * ni : nullptr);
     gj_invert<RPT, CPT>(a, ni, Dinv + l * nn, sh, m + 1);
     if (tid == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
 }
@@ -274,121 +623,162 @@ __global__ void k_zero_int(int n, int *__restrict__ p) {
     if (i < n) p[i] = 0;
 }
 
-// out_sh[i] = sum_m A[m][i] v[m] (A stored transposed), all threads; two
-// threads per row for ni <= 128, four independent accumulators per thread.
-// Ends with a barrier: out_sh is ready for every thread.
-__device__ __forceinline__ void matvec_T(int ni, const double *__restrict__ A, const double *v, double *out_sh,
-                                         double *part_sh) {
-    const int tid = threadIdx.x;
-    if (ni <= 128) {
-        const int i = tid & 127, h = tid >> 7;
-        const int half = (ni + 1) >> 1;
-        const int m0 = h * half, m1 = min(ni, m0 + half);
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        if (i < ni) {
-            int m = m0;
-            for (; m + 3 < m1; m += 4) {
-                a0 = fma(A[(size_t)m * ni + i], v[m], a0);
-                a1 = fma(A[(size_t)(m + 1) * ni + i], v[m + 1], a1);
-                a2 = fma(A[(size_t)(m + 2) * ni + i], v[m + 2], a2);
-                a3 = fma(A[(size_t)(m + 3) * ni + i], v[m + 3], a3);
-            }
-            for (; m < m1; m++) a0 = fma(A[(size_t)m * ni + i], v[m], a0);
-        }
-        const double acc = (a0 + a1) + (a2 + a3);
-        if (h == 1 && i < ni) part_sh[i] = acc;
-        __syncthreads();
-        if (h == 0 && i < ni) out_sh[i] = acc + part_sh[i];
-        __syncthreads();
-    } else {
-        for (int i = tid; i < ni; i += NT) {
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int m = 0;
-            for (; m + 3 < ni; m += 4) {
-                a0 = fma(A[(size_t)m * ni + i], v[m], a0);
-                a1 = fma(A[(size_t)(m + 1) * ni + i], v[m + 1], a1);
-                a2 = fma(A[(size_t)(m + 2) * ni + i], v[m + 2], a2);
-                a3 = fma(A[(size_t)(m + 3) * ni + i], v[m + 3], a3);
-            }
-            for (; m < ni; m++) a0 = fma(A[(size_t)m * ni + i], v[m], a0);
-            out_sh[i] = (a0 + a1) + (a2 + a3);
-        }
-        __syncthreads();
+// ---------------------------------------------------------------------------
+// Twisted block solves, 1024-thread workgroups.  Thread (i, h) = (tid % RP,
+// tid / RP) owns species row i and the h-th of SP column chunks of each
+// inverse (RP = 128 or 256 rows, SP = 8 or 4 parts), so one layer's
+// matrix-vector product is SP-way split and the
+// Dinv chunks of the next three layers are in flight while the current layer
+// is processed (the chain over layers is latency bound, not bandwidth bound).
+// Both sweeps stay within the column: layer indices run over [0, nz).
+// ---------------------------------------------------------------------------
+constexpr int NTS = 1024;
+
+// rows per part (a power of two >= 16 NB) and column chunk per thread
+template <int NB>
+struct SolveShape {
+    static constexpr int RP = 16 * NB <= 128 ? 128 : 256;
+    static constexpr int SP = NTS / RP;
+    static constexpr int CH = (16 * NB + SP - 1) / SP;
+};
+
+template <int CH>
+__device__ __forceinline__ void load_chunk(double (&r)[CH], const double *__restrict__ Xt, int ni, int i, int m0) {
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+        const int mm = m0 + q;
+        r[q] = (i < ni && mm < ni) ? Xt[(size_t)mm * ni + i] : 0.0;
     }
 }
 
-// forward halves of the twisted solve: z_j (j < m) and w_j (j > m) into tmp
-__global__ void __launch_bounds__(NT) k_solve_fwd(int ni, const int *__restrict__ off,
-                                                   const double *__restrict__ Dinv, const double *__restrict__ Lsub,
-                                                   const double *__restrict__ Usup, const double *in,
-                                                   double *tmp, const vk_colstate *__restrict__ st) {
+// layers in flight per thread: fewer for larger blocks (128 registers per thread at 1024 threads)
+template <int NB>
+struct SolveDepth {
+    static constexpr int value = NB <= 5 ? 3 : (NB <= 7 ? 2 : 1);
+};
+
+template <int DEP, int CH>
+__device__ __forceinline__ void rotate(double (&b)[DEP][CH]) {
+#pragma unroll
+    for (int d = 0; d + 1 < DEP; d++) {
+#pragma unroll
+        for (int q = 0; q < CH; q++) b[d][q] = b[d + 1][q];
+    }
+}
+
+// y_i = sum_m Xt[m][i] v[m] from the registers of the eight row parts; the
+// result is valid in threads h == 0 (i < ni).  Two barriers.
+template <int SP, int RP, int CH>
+__device__ __forceinline__ double solve_matvec(const double (&cur)[CH], const double *v, double (*part)[RP], int i,
+                                               int h, int m0) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < CH; q++) acc = fma(cur[q], v[m0 + q], acc);
+    part[h][i] = acc;
+    __syncthreads();
+    double y = part[0][i];
+#pragma unroll
+    for (int hh = 1; hh < SP; hh++) y += part[hh][i];
+    return y;
+}
+
+template <int NB>
+__global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restrict__ off,
+                                                     const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                                                     const double *__restrict__ Usup, const double *in, double *tmp,
+                                                     const vk_colstate *__restrict__ st) {
+    constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
     const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
     if (st && st[c].done) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double v[VK_NI_MAX];
-    __shared__ double z[VK_NI_MAX];
-    __shared__ double part[VK_NI_MAX];
-    const int j0 = side == 0 ? 0 : nz - 1, j1 = side == 0 ? m : m, dj = side == 0 ? 1 : -1;
-    for (int j = j0; j != j1; j += dj) {
-        const size_t l = (size_t)l0 + j;
-        const bool first = (j == j0);
-        const double *cpl = side == 0 ? Lsub + l * ni : Usup + l * ni;
-        for (int i = tid; i < ni; i += NT) {
+    __shared__ double v[SP * CH];
+    __shared__ double part[SP][RP];
+    for (int q = tid; q < SP * CH; q += NTS) v[q] = 0.0;
+    const int j0 = side == 0 ? 0 : nz - 1, dj = side == 0 ? 1 : -1;
+    const int nsteps = side == 0 ? m : nz - 1 - m;
+    const double *cpl = side == 0 ? Lsub : Usup;
+    constexpr int DEP = SolveDepth<NB>::value;
+    double buf[DEP][CH];
+#pragma unroll
+    for (int d = 0; d < DEP; d++)
+        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)(l0 + j0 + d * dj) * nn, ni, i, m0);
+    double zi = 0.0;
+    __syncthreads();
+    for (int s = 0; s < nsteps; s++) {
+        const size_t l = (size_t)(l0 + j0 + dj * s);
+        if (h == 0 && i < ni) {
             double w = in[l * ni + i];
-            if (!first) w = w - cpl[i] * z[i];
+            if (s > 0) w = w - cpl[l * ni + i] * zi;
             v[i] = w;
         }
         __syncthreads();
-        matvec_T(ni, Dinv + l * nn, v, z, part);
-        for (int i = tid; i < ni; i += NT) tmp[l * ni + i] = z[i];
+        const double z = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
+        if (h == 0 && i < ni) {
+            tmp[l * ni + i] = z;
+            zi = z;
+        }
+        rotate<DEP, CH>(buf);
+        if (s + DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)(l0 + j0 + dj * (s + DEP)) * nn, ni, i, m0);
+        __syncthreads();
     }
     // park b_m in the unused slot m so the backward kernel never reads `in`
     // (which may alias `out`)
-    if (side == 0) {
+    if (side == 0 && h == 0 && i < ni) {
         const size_t lm = (size_t)l0 + m;
-        for (int i = tid; i < ni; i += NT) tmp[lm * ni + i] = in[lm * ni + i];
+        tmp[lm * ni + i] = in[lm * ni + i];
     }
 }
 
-// middle solution and back substitution outwards: x into out
-__global__ void __launch_bounds__(NT) k_solve_bwd(int ni, const int *__restrict__ off,
-                                                   const double *__restrict__ Dinv, const double *__restrict__ Lsub,
-                                                   const double *__restrict__ Usup, const double *tmp, double *out,
-                                                   const vk_colstate *__restrict__ st) {
+template <int NB>
+__global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restrict__ off,
+                                                     const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                                                     const double *__restrict__ Usup, const double *tmp, double *out,
+                                                     const vk_colstate *__restrict__ st) {
+    constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
     const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
     if (st && st[c].done) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double v[VK_NI_MAX];
-    __shared__ double x[VK_NI_MAX];
-    __shared__ double y[VK_NI_MAX];
-    __shared__ double part[VK_NI_MAX];
+    __shared__ double v[SP * CH];
+    __shared__ double part[SP][RP];
+    for (int q = tid; q < SP * CH; q += NTS) v[q] = 0.0;
     const size_t lm = (size_t)l0 + m;
+    const int dj = side == 0 ? -1 : 1;
+    const int nsteps = 1 + (side == 0 ? m : nz - 1 - m);     // the middle layer, then the chain outwards
+    const double *cpl = side == 0 ? Usup : Lsub;
+    constexpr int DEP = SolveDepth<NB>::value;
+    double buf[DEP][CH];
+#pragma unroll
+    for (int d = 0; d < DEP; d++)
+        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)((long)lm + (long)d * dj) * nn, ni, i, m0);
     // x_m = M^-1 (b_m - Lsub_m z_{m-1} - Usup_m w_{m+1}); both sides compute it
-    for (int i = tid; i < ni; i += NT) {
-        double w = tmp[lm * ni + i] - Lsub[lm * ni + i] * tmp[(lm - 1) * ni + i];
+    if (h == 0 && i < ni) {
+        double w = tmp[lm * ni + i];
+        if (m > 0) w = w - Lsub[lm * ni + i] * tmp[(lm - 1) * ni + i];
         if (m < nz - 1) w = w - Usup[lm * ni + i] * tmp[(lm + 1) * ni + i];
         v[i] = w;
     }
     __syncthreads();
-    matvec_T(ni, Dinv + lm * nn, v, x, part);
-    if (side == 0)
-        for (int i = tid; i < ni; i += NT) out[lm * ni + i] = x[i];
-    const int j0 = side == 0 ? m - 1 : m + 1, j1 = side == 0 ? -1 : nz, dj = side == 0 ? -1 : 1;
-    for (int j = j0; j != j1; j += dj) {
-        const size_t l = (size_t)l0 + j;
-        const double *cpl = side == 0 ? Usup + l * ni : Lsub + l * ni;
-        for (int i = tid; i < ni; i += NT) v[i] = cpl[i] * x[i];
+    double xi = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
+    if (side == 0 && h == 0 && i < ni) out[lm * ni + i] = xi;
+    rotate<DEP, CH>(buf);
+    if (DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)DEP * dj) * nn, ni, i, m0);
+    __syncthreads();
+    for (int s = 1; s < nsteps; s++) {
+        const size_t l = lm + (long)dj * s;
+        if (h == 0 && i < ni) v[i] = cpl[l * ni + i] * xi;
         __syncthreads();
-        matvec_T(ni, Dinv + l * nn, v, y, part);
-        for (int i = tid; i < ni; i += NT) {
-            const double xi = tmp[l * ni + i] - y[i];
+        const double y = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
+        if (h == 0 && i < ni) {
+            xi = tmp[l * ni + i] - y;
             out[l * ni + i] = xi;
-            x[i] = xi;
         }
+        rotate<DEP, CH>(buf);
+        if (s + DEP < nsteps)
+            load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)dj * (s + DEP)) * nn, ni, i, m0);
         __syncthreads();
     }
 }
@@ -406,24 +796,58 @@ __device__ inline void dd_sub_prod(double &hi, double &lo, double a, double b) {
     lo = lo + (se - pe);
 }
 
+// r = b - A x, 32 rows of one layer per workgroup: the rows of D are staged
+// in LDS with coalesced loads, 8 threads share a row (each a double-double
+// partial over an eighth of the columns), and the partials, b and the two
+// coupling terms are combined in double-double.
+constexpr int RES_ROWS = 32, RES_PARTS = NT / RES_ROWS;
+
+__device__ inline void dd_add(double &hi, double &lo, double a_hi, double a_lo) {
+    const double s = hi + a_hi;
+    const double bb = s - hi;
+    const double e = (hi - (s - bb)) + (a_hi - bb);     // hi + a_hi = s + e exactly
+    hi = s;
+    lo = lo + (e + a_lo);
+}
+
 __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__ off,
                                                   const int *__restrict__ layer_col, const double *__restrict__ D,
                                                   const double *__restrict__ Lsub, const double *__restrict__ Usup,
                                                   const double *__restrict__ x, const double *__restrict__ b,
                                                   double *__restrict__ r, const vk_colstate *__restrict__ st) {
+    extern __shared__ double rsm[];
+    double *Ds = rsm;                        // RES_ROWS x ni
+    double *xs = rsm + RES_ROWS * ni;        // ni
+    double *ph = xs + ni;                    // RES_ROWS x RES_PARTS partial sums (hi)
+    double *pl = ph + NT;                    // (lo)
     const int l = blockIdx.x;
     const int c = layer_col[l];
     if (st && st[c].done) return;
     const int j = l - off[c], nz = off[c + 1] - off[c];
-    const double *Dl = D + (size_t)l * ni * ni;
-    const double *xl = x + (size_t)l * ni;
-    for (int i = threadIdx.x; i < ni; i += NT) {
-        double hi = b[(size_t)l * ni + i], lo = 0.0;
-        const double *row = Dl + (size_t)i * ni;
-        for (int m = 0; m < ni; m++) dd_sub_prod(hi, lo, row[m], xl[m]);
-        if (j > 0) dd_sub_prod(hi, lo, Lsub[(size_t)l * ni + i], x[(size_t)(l - 1) * ni + i]);
-        if (j < nz - 1) dd_sub_prod(hi, lo, Usup[(size_t)l * ni + i], x[(size_t)(l + 1) * ni + i]);
-        r[(size_t)l * ni + i] = hi + lo;
+    const int i0 = blockIdx.y * RES_ROWS;
+    const int nrows = min(RES_ROWS, ni - i0);
+    const double *Dl = D + ((size_t)l * ni + i0) * ni;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nrows * ni; e += NT) Ds[e] = Dl[e];
+    for (int m = tid; m < ni; m += NT) xs[m] = x[(size_t)l * ni + m];
+    __syncthreads();
+    const int rr = tid / RES_PARTS, pp = tid % RES_PARTS;
+    const int chunk = (ni + RES_PARTS - 1) / RES_PARTS;
+    double hi = 0.0, lo = 0.0;
+    if (rr < nrows) {
+        const int m1 = min(ni, (pp + 1) * chunk);
+        for (int m = pp * chunk; m < m1; m++) dd_sub_prod(hi, lo, Ds[rr * ni + m], xs[m]);
+    }
+    ph[tid] = hi;
+    pl[tid] = lo;
+    __syncthreads();
+    if (pp == 0 && rr < nrows) {
+        const int i = i0 + rr;
+        double h = b[(size_t)l * ni + i], o = 0.0;
+        for (int q = 0; q < RES_PARTS; q++) dd_add(h, o, ph[tid + q], pl[tid + q]);
+        if (j > 0) dd_sub_prod(h, o, Lsub[(size_t)l * ni + i], x[(size_t)(l - 1) * ni + i]);
+        if (j < nz - 1) dd_sub_prod(h, o, Usup[(size_t)l * ni + i], x[(size_t)(l + 1) * ni + i]);
+        r[(size_t)l * ni + i] = h + o;
     }
 }
 
@@ -436,11 +860,23 @@ __global__ void k_axpy(int n, int ni, const int *__restrict__ layer_col, const d
 }
 
 template <int N>
-int launch_factor_n(int ni, int ncols, const int *off, const double *D, const double *Lsub, const double *Usup,
-                    double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_factor<N, N>), dim3(2 * ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info, st);
-    hipLaunchKernelGGL((k_factor_mid<N, N>), dim3(ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info, st);
-    return 0;
+void launch_factor_n(int variant, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+                     const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
+    if (variant == VK_FACTOR_BLOCKED) {
+        hipLaunchKernelGGL((k_factor_blk<N, 1>), dim3(2 * ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+                           st);
+        hipLaunchKernelGGL((k_factor_blk_mid<N, 1>), dim3(ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+                           st);
+    } else if (variant == VK_FACTOR_BLOCKED_OCC2) {
+        hipLaunchKernelGGL((k_factor_blk<N, 2>), dim3(2 * ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+                           st);
+        hipLaunchKernelGGL((k_factor_blk_mid<N, 2>), dim3(ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+                           st);
+    } else {
+        hipLaunchKernelGGL((k_factor<N, N>), dim3(2 * ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info, st);
+        hipLaunchKernelGGL((k_factor_mid<N, N>), dim3(ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+                           st);
+    }
 }
 
 }  // namespace
@@ -450,18 +886,19 @@ int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *
     const int ni = plan->ni;
     const int n = (ni + 15) / 16;
     const int nc = cols->ncols;
+    const int v = plan->factor_variant;
     if (info) hipLaunchKernelGGL(k_zero_int, dim3((nc + 255) / 256), dim3(256), 0, s, nc, info);
     switch (n) {
-    case 1: launch_factor_n<1>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 2: launch_factor_n<2>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 3: launch_factor_n<3>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 4: launch_factor_n<4>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 5: launch_factor_n<5>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 6: launch_factor_n<6>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 7: launch_factor_n<7>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 8: launch_factor_n<8>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 9: launch_factor_n<9>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 10: launch_factor_n<10>(ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 1: launch_factor_n<1>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 2: launch_factor_n<2>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 3: launch_factor_n<3>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 4: launch_factor_n<4>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 5: launch_factor_n<5>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 6: launch_factor_n<6>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 7: launch_factor_n<7>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 8: launch_factor_n<8>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 9: launch_factor_n<9>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 10: launch_factor_n<10>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
     default: return VK_ETOOBIG;
     }
     VK_CHECK(hipGetLastError());
@@ -469,13 +906,33 @@ int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *
 }
 
 // out = lhs^-1 in (tmp: L*ni scratch, may not alias in/out; in may alias out)
+namespace {
+template <int NB>
+void launch_solve_n(int ni, int ncols, const int *off, const double *Dinv, const double *Lsub, const double *Usup,
+                    const double *in, double *out, double *tmp, const vk_colstate *st, hipStream_t s) {
+    hipLaunchKernelGGL((k_solve_fwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, in, tmp, st);
+    hipLaunchKernelGGL((k_solve_bwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, tmp, out, st);
+}
+}  // namespace
+
 int vk_launch_solve(const vk_plan *plan, const vk_columns *cols, const double *Dinv, const double *Lsub,
                     const double *Usup, const double *in, double *out, double *tmp, const vk_colstate *st,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_solve_fwd, dim3(2 * cols->ncols), dim3(NT), 0, s, plan->ni, cols->d_off, Dinv, Lsub, Usup, in,
-                       tmp, st);
-    hipLaunchKernelGGL(k_solve_bwd, dim3(2 * cols->ncols), dim3(NT), 0, s, plan->ni, cols->d_off, Dinv, Lsub, Usup,
-                       tmp, out, st);
+    const int ni = plan->ni, nc = cols->ncols;
+    const int *off = cols->d_off;
+    switch ((ni + 15) / 16) {
+    case 1: launch_solve_n<1>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 2: launch_solve_n<2>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 3: launch_solve_n<3>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 4: launch_solve_n<4>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 5: launch_solve_n<5>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 6: launch_solve_n<6>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 7: launch_solve_n<7>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 8: launch_solve_n<8>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 9: launch_solve_n<9>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 10: launch_solve_n<10>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    default: return VK_ETOOBIG;
+    }
     VK_CHECK(hipGetLastError());
     return 0;
 }
@@ -483,8 +940,10 @@ int vk_launch_solve(const vk_plan *plan, const vk_columns *cols, const double *D
 int vk_launch_residual(const vk_plan *plan, const vk_columns *cols, const double *D, const double *Lsub,
                        const double *Usup, const double *x, const double *b, double *r, const vk_colstate *st,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_residual, dim3(cols->L), dim3(NT), 0, s, plan->ni, cols->d_off, cols->d_layer_col, D, Lsub,
-                       Usup, x, b, r, st);
+    const int ni = plan->ni;
+    const size_t smem = sizeof(double) * ((size_t)RES_ROWS * ni + ni + 2 * NT);
+    hipLaunchKernelGGL(k_residual, dim3(cols->L, (ni + RES_ROWS - 1) / RES_ROWS), dim3(NT), smem, s, ni, cols->d_off,
+                       cols->d_layer_col, D, Lsub, Usup, x, b, r, st);
     VK_CHECK(hipGetLastError());
     return 0;
 }
@@ -531,4 +990,96 @@ extern "C" int vk_block_residual(const vk_plan *plan, const vk_columns *cols, co
                                  double *d_r, void *stream) {
     if (!plan || !cols || !d_D || !d_Lsub || !d_Usup || !d_x || !d_b || !d_r) return VK_EINVAL;
     return vk_launch_residual(plan, cols, d_D, d_Lsub, d_Usup, d_x, d_b, d_r, nullptr, (hipStream_t)stream);
+}
+
+namespace {
+// a well-conditioned block system of the columns' shape for timing
+__global__ void k_fill_tune(int ni, long n_d, double *__restrict__ D, long n_c, double *__restrict__ Lsub,
+                            double *__restrict__ Usup) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_d) {
+        const long e = t % ((long)ni * ni);
+        const int i = (int)(e / ni), j = (int)(e % ni);
+        D[t] = (i == j) ? 2.0 * ni : 1.0 / (1.0 + i + 2.0 * j);
+    }
+    if (t < n_c) {
+        Lsub[t] = -0.25;
+        Usup[t] = -0.5;
+    }
+}
+}  // namespace
+
+extern "C" int vk_plan_set_factor_variant(vk_plan *plan, int variant) {
+    if (!plan || variant < 0 || variant >= VK_FACTOR_NVARIANTS) return VK_EINVAL;
+    plan->factor_variant = variant;
+    return 0;
+}
+
+extern "C" int vk_plan_factor_variant(const vk_plan *plan) { return plan ? plan->factor_variant : VK_EINVAL; }
+
+extern "C" int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, float *ms, void *stream) {
+    if (!plan || !cols || reps < 1 || !ms) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)plan->ni * plan->ni, L = (size_t)cols->L;
+    double *D = nullptr, *Dinv = nullptr, *Lsub = nullptr, *Usup = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMallocAsync((void **)&D, sizeof(double) * L * nn, s) != hipSuccess ||
+        hipMallocAsync((void **)&Dinv, sizeof(double) * L * nn, s) != hipSuccess ||
+        hipMallocAsync((void **)&Lsub, sizeof(double) * L * plan->ni, s) != hipSuccess ||
+        hipMallocAsync((void **)&Usup, sizeof(double) * L * plan->ni, s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = VK_ENOMEM;
+    } else {
+        const long nd = (long)(L * nn);
+        hipLaunchKernelGGL(k_fill_tune, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, plan->ni, nd, D,
+                           (long)(L * plan->ni), Lsub, Usup);
+        const int keep = plan->factor_variant;
+        int bestv = keep;
+        float bestms = 0.f;
+        for (int v = 0; v < VK_FACTOR_NVARIANTS && rc == 0; v++) {
+            plan->factor_variant = v;
+            rc = vk_launch_factor(plan, cols, D, Lsub, Usup, Dinv, nullptr, nullptr, s);   // warm-up
+            if (rc) break;
+            hipError_t e = hipEventRecord(e0, s);
+            if (e != hipSuccess) { rc = -(int)e; break; }
+            for (int q = 0; q < reps && rc == 0; q++) rc = vk_launch_factor(plan, cols, D, Lsub, Usup, Dinv, nullptr, nullptr, s);
+            if (rc) break;
+            e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e != hipSuccess) { rc = -(int)e; break; }
+            ms[v] = t / reps;
+            if (v == 0 || ms[v] < bestms) { bestms = ms[v]; bestv = v; }
+        }
+        plan->factor_variant = rc ? keep : bestv;
+    }
+    if (D) (void)hipFreeAsync(D, s);
+    if (Dinv) (void)hipFreeAsync(Dinv, s);
+    if (Lsub) (void)hipFreeAsync(Lsub, s);
+    if (Usup) (void)hipFreeAsync(Usup, s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
+namespace {
+__global__ void k_selftest_wave_max(const unsigned long long *in, unsigned long long *out) {
+    out[threadIdx.x] = wave_max_u64(in[threadIdx.x]);
+}
+}  // namespace
+
+extern "C" int vk_selftest_wave_max_u64(const unsigned long long *d_in, unsigned long long *d_out, void *stream) {
+    if (!d_in || !d_out) return VK_EINVAL;
+    hipLaunchKernelGGL(k_selftest_wave_max, dim3(1), dim3(64), 0, (hipStream_t)stream, d_in, d_out);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vk_selftest_mfma_f64(double *d_out, void *stream) {
+    if (!d_out) return VK_EINVAL;
+    hipLaunchKernelGGL(k_selftest_mfma_f64, dim3(1), dim3(64), 0, (hipStream_t)stream, d_out);
+    VK_CHECK(hipGetLastError());
+    return 0;
 }
