@@ -166,15 +166,19 @@ int vk_block_factor(const vk_plan *plan, const vk_columns *cols, const double *d
  * the block in registers and runs one barrier-synchronised Gauss-Jordan
  * step per pivot; VK_FACTOR_BLOCKED groups the pivots in panels of 16,
  * factors each panel in one wave and applies the rank-16 Schur update on
- * FP64 MFMA (compiled for one or for two resident workgroups per CU).  All
- * give the inverse to rounding; vk_tune_factor times every
+ * FP64 MFMA (compiled for one or for two resident workgroups per CU);
+ * VK_FACTOR_WAVE holds the block in the registers of two waves and runs
+ * one two-wave barrier per pivot step (small workgroups: several chains per
+ * CU).  All give the inverse to rounding; vk_tune_factor times every
  * variant on a synthetic system of the columns' shape (`reps` launches
- * each, milliseconds per launch into ms[VK_FACTOR_NVARIANTS]) and keeps the
+ * each, milliseconds per launch into ms[VK_FACTOR_NVARIANTS], -1 for a
+ * variant the plan's ni does not allow) and keeps the
  * fastest in the plan. */
 #define VK_FACTOR_REGISTER     0
 #define VK_FACTOR_BLOCKED      1   /* register budget for one workgroup per CU      */
 #define VK_FACTOR_BLOCKED_OCC2 2   /* register budget for two workgroups per CU     */
-#define VK_FACTOR_NVARIANTS    3
+#define VK_FACTOR_WAVE         3   /* two waves per block, unblocked (ni <= 112)    */
+#define VK_FACTOR_NVARIANTS    4
 int vk_plan_set_factor_variant(vk_plan *plan, int variant);
 int vk_plan_factor_variant(const vk_plan *plan);
 int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, float *ms, void *stream);

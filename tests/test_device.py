@@ -182,7 +182,7 @@ def test_wave_max_u64():
         assert (out.cpu().numpy() == v.max()).all(), trial
 
 
-FACTOR_VARIANTS = [0, 1, 2]      # register Gauss-Jordan, blocked MFMA (1 or 2 workgroups per CU)
+FACTOR_VARIANTS = [0, 1, 2, 3]   # register Gauss-Jordan, blocked MFMA (1 or 2 workgroups per CU), two-wave
 
 
 @gpu
@@ -190,6 +190,8 @@ FACTOR_VARIANTS = [0, 1, 2]      # register Gauss-Jordan, blocked MFMA (1 or 2 w
 @pytest.mark.parametrize('ni,nz', [(30, 60), (93, 150), (150, 12), (7, 3), (16, 2), (40, 3), (17, 5)])
 def test_block_factor_solve_against_dense(ni, nz, variant):
     torch = _torch()
+    if variant == 3 and ni > 112:
+        pytest.skip('the two-wave variant holds blocks up to ni = 112')
     from vulcan_amd.device import DeviceColumns, DevicePlan, dptr, stream_handle
     rng = np.random.default_rng(ni * 1000 + nz)
     D = rng.standard_normal((nz, ni, ni))
@@ -246,7 +248,7 @@ def test_zero_pivot_is_reported():
     L = np.zeros((nz, ni)); U = np.zeros((nz, ni))
     lib = _lib.load()
     for variant in FACTOR_VARIANTS:
-        plan = DevicePlan(_sized_network(ni))
+        plan = DevicePlan(_sized_network(ni))   # ni = 20: every variant applies
         _lib.check(lib.vk_plan_set_factor_variant(plan.handle, variant), 'vk_plan_set_factor_variant')
         cols = DeviceColumns([nz])
         T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device='cuda')
@@ -270,8 +272,14 @@ def test_factor_tuner_picks_the_fastest_variant():
     _lib.check(lib.vk_tune_factor(plan.handle, cols.handle, 2, ms, stream_handle()), 'vk_tune_factor')
     t = list(ms)
     print('factor variants ms:', t)
-    assert all(v > 0 for v in t)
+    assert all(v > 0 for v in t)                       # ni = 93: every variant applies
     assert lib.vk_plan_factor_variant(plan.handle) == int(np.argmin(t))
+    big = DevicePlan(_sized_network(150))
+    big_cols = DeviceColumns([7])
+    ms2 = (C.c_float * len(FACTOR_VARIANTS))()
+    _lib.check(lib.vk_tune_factor(big.handle, big_cols.handle, 1, ms2, stream_handle()), 'vk_tune_factor')
+    assert ms2[3] == -1.0 and all(ms2[v] > 0 for v in (0, 1, 2))
+    assert lib.vk_plan_set_factor_variant(big.handle, 3) == -1001        # VK_EINVAL
 
 
 @gpu

@@ -124,35 +124,39 @@ __device__ inline LayerCtx layer_ctx(const int *off, const int *layer_col, int l
     return x;
 }
 
-// v_p = k_f * prod(reac) - k_r * prod(prod), left to right (make_chem_funs.py:293-315)
-__device__ inline double pair_rate(const int *reac, const int *prod, const double *kl, const double *yl,
+// v_p = k_f * prod(reac) - k_r * prod(prod), left to right (make_chem_funs.py:293-315);
+// the four terms of a side are one 16-byte load
+__device__ inline double side_product(int4 t4, double kcol, const double *yl, double m) {
+    const int t[4] = {t4.x, t4.y, t4.z, t4.w};
+    double a = kcol;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (t[q] == -2) continue;
+        a = a * (t[q] == -1 ? m : yl[t[q]]);
+    }
+    return a;
+}
+
+__device__ inline double pair_rate(const int4 *reac, const int4 *prod, const double *kl, const double *yl,
                                    double m, int p) {
-    double a = kl[2 * p];
-    for (int q = 0; q < 4; q++) {
-        int t = reac[4 * p + q];
-        if (t == -2) continue;
-        a = a * (t == -1 ? m : yl[t]);
-    }
-    double b = kl[2 * p + 1];
-    for (int q = 0; q < 4; q++) {
-        int t = prod[4 * p + q];
-        if (t == -2) continue;
-        b = b * (t == -1 ? m : yl[t]);
-    }
+    const double a = side_product(reac[p], kl[2 * p], yl, m);
+    const double b = side_product(prod[p], kl[2 * p + 1], yl, m);
     return a - b;
 }
 
-// d/dy_m of kcol * prod(terms)
-__device__ inline double dside(const int *terms, double kcol, const double *yl, double m, int sp, bool &any) {
+// d/dy_sp of kcol * prod(terms)
+__device__ inline double dside(int4 t4, double kcol, const double *yl, double m, int sp, bool &any) {
+    const int t[4] = {t4.x, t4.y, t4.z, t4.w};
     double tot = 0.0;
     any = false;
+#pragma unroll
     for (int q = 0; q < 4; q++) {
-        if (terms[q] != sp) continue;
+        if (t[q] != sp) continue;
         double acc = kcol;
+#pragma unroll
         for (int q2 = 0; q2 < 4; q2++) {
-            int t2 = terms[q2];
-            if (q2 == q || t2 == -2) continue;
-            acc = acc * (t2 == -1 ? m : yl[t2]);
+            if (q2 == q || t[q2] == -2) continue;
+            acc = acc * (t[q2] == -1 ? m : yl[t[q2]]);
         }
         tot = any ? tot + acc : acc;
         any = true;
@@ -193,7 +197,8 @@ __global__ void __launch_bounds__(BLK) k_rhs(
     if (mode & 1) {
         const double *kl = k + (size_t)l * 2 * npairs;
         double m = M[l];
-        for (int p = threadIdx.x; p < npairs; p += blockDim.x) v[p] = pair_rate(reac, prod, kl, yl, m, p);
+        for (int p = threadIdx.x; p < npairs; p += blockDim.x)
+            v[p] = pair_rate((const int4 *)reac, (const int4 *)prod, kl, yl, m, p);
         __syncthreads();
     }
     ColAtm ca = col_atm(atm, x.off, x.nz, ni);
@@ -228,8 +233,8 @@ __global__ void __launch_bounds__(BLK) k_rhs(
 __global__ void __launch_bounds__(BLK) k_jac_lhs(
     int ni, int npairs, int nslot, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
-    const int *__restrict__ slot_sp, const int2 *__restrict__ jac_rng, const int *__restrict__ jac_slot,
-    const double *__restrict__ jac_coef, const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm,
+    const int *__restrict__ slot_sp, const int2 *__restrict__ jac_rng, const vk_jac_term *__restrict__ jac_term,
+    const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm,
     const double *__restrict__ y, const double *__restrict__ k, const double *__restrict__ M, int mode, int mol,
     const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ D,
     double *__restrict__ Lsub, double *__restrict__ Usup) {
@@ -256,8 +261,8 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
         int p = slot_pair[s], sp = slot_sp[s];
         bool a1, a2;
-        double dr = dside(reac + 4 * p, kl[2 * p], yl, m, sp, a1);
-        double dp = dside(prod + 4 * p, kl[2 * p + 1], yl, m, sp, a2);
+        double dr = dside(((const int4 *)reac)[p], kl[2 * p], yl, m, sp, a1);
+        double dp = dside(((const int4 *)prod)[p], kl[2 * p + 1], yl, m, sp, a2);
         dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
     }
     double c0 = 0.0;
@@ -283,8 +288,12 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         const int2 rg = jac_rng[e];
         double J = 0.0;
         if (rg.y > rg.x) {
-            J = jac_coef[rg.x] * dv[jac_slot[rg.x]];
-            for (int q = rg.x + 1; q < rg.y; q++) J = J + jac_coef[q] * dv[jac_slot[q]];
+            vk_jac_term t = jac_term[rg.x];
+            J = t.coef * dv[t.slot];
+            for (int q = rg.x + 1; q < rg.y; q++) {
+                t = jac_term[q];
+                J = J + t.coef * dv[t.slot];
+            }
         }
         if (mode == 0) {
             Dl[e] = J;
@@ -345,7 +354,7 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     if (atm) a = *atm;
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair, plan->slot_sp,
-                       plan->jac_rng, plan->jac_slot, plan->jac_coef, plan->ms, plan->alpha, a, y, k, M, mode, mol,
+                       plan->jac_rng, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
                        dtc, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;

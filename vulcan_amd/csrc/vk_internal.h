@@ -14,6 +14,12 @@
     } while (0)
 
 
+// one term of a Jacobian entry: coef * dv[slot] (one 16-byte load)
+struct vk_jac_term {
+    double coef;
+    int slot, pad;
+};
+
 struct vk_plan {
     int ni, npairs, nr, natom, nchem, nslot, njnz, njac;
     int factor_variant;
@@ -23,6 +29,7 @@ struct vk_plan {
     int *slot_pair, *slot_sp;
     int *jac_map, *jac_ptr, *jac_slot;
     int2 *jac_rng;             // per dense entry: [jac_ptr[jm], jac_ptr[jm+1]) or empty
+    vk_jac_term *jac_term;     // jac_coef / jac_slot interleaved
     double *jac_coef;
     std::vector<void *> allocs;
 };

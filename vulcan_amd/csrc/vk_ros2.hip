@@ -445,6 +445,11 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
         }
         r = upload(v, &p->jac_rng, rng.data(), rng.size());
     }
+    if (!r) {
+        std::vector<vk_jac_term> terms((size_t)n->njac);
+        for (size_t q = 0; q < terms.size(); q++) terms[q] = vk_jac_term{n->jac_coef[q], n->jac_slot[q], 0};
+        r = upload(v, &p->jac_term, terms.data(), terms.size());
+    }
     if (r) { vk_plan_destroy(p); return r; }
     *out = p;
     return 0;

@@ -26,6 +26,9 @@ def _require_device():
         raise _lib.VulcanAmdError('no HIP device visible: the vulcan_amd kernels need an MI355X (gfx950)')
 
 
+FACTOR_NVARIANTS = 4     # VK_FACTOR_NVARIANTS
+
+
 def stream_handle(stream: Optional[torch.cuda.Stream] = None) -> C.c_void_p:
     s = stream or torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
@@ -147,7 +150,7 @@ class ColumnBatch:
     """
 
     def __init__(self, plan: DevicePlan, cols: List[Column], y0: Sequence[np.ndarray], cfg: Config,
-                 history: int = 0, k: Optional[np.ndarray] = None, graph: bool = False):
+                 history: int = 0, k: Optional[np.ndarray] = None, graph: bool = False, tune: bool = True):
         cfg.check_supported()
         self.plan, self.cfg = plan, cfg
         net = plan.net
@@ -215,6 +218,7 @@ class ColumnBatch:
         else:
             self.k.copy_(torch.as_tensor(k, dtype=F64))
         self.reset_element_baseline()
+        self.factor_ms = self.tune() if tune else None
 
     # -- setup ------------------------------------------------------------------
     def compute_rates(self):
@@ -246,6 +250,15 @@ class ColumnBatch:
                                           dptr(self.k), dptr(self.y), dptr(self.ymix), dptr(self.y_prev),
                                           dptr(self.atom_ini), dptr(self.atom_loss), dptr(self.atom_loss_prev),
                                           dptr(self.state), int(n_attempts), stream_handle()), 'vk_integrate')
+
+    def tune(self, reps: int = 2):
+        """Time every block-inversion variant on this batch's shape and keep
+        the fastest in the plan; returns milliseconds per variant (-1: not
+        applicable)."""
+        ms = (C.c_float * FACTOR_NVARIANTS)()
+        _lib.check(self._lib.vk_tune_factor(self.plan.handle, self.layout.handle, int(reps), ms, stream_handle()),
+                   'vk_tune_factor')
+        return list(ms)
 
     def set_graph(self, on: bool):
         """Replay a captured HIP graph per attempt in ``integrate``."""

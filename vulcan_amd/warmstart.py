@@ -43,18 +43,18 @@ class ColumnSpec:
     logab: np.ndarray      # (4,) log10 of C/H, O/H, N/H, He/H
 
 
-def sample_specs(n: int, seed: int) -> List[ColumnSpec]:
+def sample_specs(n: int, seed: int, nz: int = NZ) -> List[ColumnSpec]:
     out = []
     for s in np.random.SeedSequence(seed).spawn(n):
         rng = np.random.default_rng(s)
         T_top, T_bot = rng.uniform(400., 2500., 2)
         logK = rng.uniform(5., 11., 3)
         logab = np.array([np.log10(SOLAR[k]) for k in ABUND_KEYS]) + rng.uniform(-1., 1., 4)
-        out.append(ColumnSpec(T=iso_gradient_profile(NZ, T_top, T_bot, P_B, P_T), logK=logK, logab=logab))
+        out.append(ColumnSpec(T=iso_gradient_profile(nz, T_top, T_bot, P_B, P_T), logK=logK, logab=logab))
     return out
 
 
-def kzz_profile(logK: np.ndarray, nz: int = NZ) -> np.ndarray:
+def kzz_profile(logK: np.ndarray, nz: int) -> np.ndarray:
     x = np.linspace(0.0, 1.0, nz - 1)
     return 10.0 ** np.interp(x, [0.0, 0.5, 1.0], logK)
 
@@ -68,17 +68,17 @@ def element_vector(net: Network, logab: np.ndarray) -> np.ndarray:
 
 def make_case(net: Network, spec: ColumnSpec, cfg: Optional[Config] = None, ymix0=None) -> ColumnCase:
     cfg = cfg or Config()
-    pco = np.logspace(np.log10(P_B), np.log10(P_T), NZ)
+    nz = len(spec.T)
     if ymix0 is None:
         ymix0 = cold_start(net, spec)
-    col = build_column(net, NZ, spec.T[0], spec.T[-1], P_b=P_B, P_t=P_T, ymix0=ymix0, cfg=cfg,
+    col = build_column(net, nz, spec.T[0], spec.T[-1], P_b=P_B, P_t=P_T, ymix0=ymix0, cfg=cfg,
                        tau_top=1e-4, tau_bot=1e2, T_profile=spec.T)
-    col.Kzz = kzz_profile(spec.logK)
+    col.Kzz = kzz_profile(spec.logK, nz)
     return ColumnCase(net=net, col=col, y0=col.M[:, None] * ymix0, cfg=cfg)
 
 
 def cold_start(net: Network, spec: ColumnSpec) -> np.ndarray:
-    pco = np.logspace(np.log10(P_B), np.log10(P_T), NZ)
+    pco = np.logspace(np.log10(P_B), np.log10(P_T), len(spec.T))
     x = equilibrium(net, spec.T, pco, element_vector(net, spec.logab))
     x = np.maximum(x, 1e-30)
     return x / x.sum(axis=1, keepdims=True)
@@ -131,7 +131,9 @@ def project(net: Network, x_pred: np.ndarray, b: np.ndarray, iters: int = 100) -
 # --------------------------------------------------------------------------
 
 def features(specs: Sequence[ColumnSpec]) -> np.ndarray:
-    logP = np.log10(np.logspace(np.log10(P_B), np.log10(P_T), NZ))
+    """T (nz), log10 P (nz), log10 Kzz at the three anchors, four log10 abundances."""
+    nz = len(specs[0].T)
+    logP = np.log10(np.logspace(np.log10(P_B), np.log10(P_T), nz))
     return np.stack([np.concatenate([s.T, logP, s.logK, s.logab]) for s in specs]).astype(np.float64)
 
 
