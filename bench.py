@@ -18,7 +18,9 @@ The JSON line also carries
   repack, two banded LAPACK solves per attempt) timed on the host in a
   single-threaded subprocess on a bounded sample;
 * ``batched``: the same network on 256 columns with nz uniform on [80, 200]
-  (one launch sequence per attempt for all columns).
+  (one captured launch sequence per attempt for all columns);
+* ``mixed``: 512 columns over three networks (ni = 40, 93, 150), one batch
+  per network on its own stream, block-inversion kernel tuned per ni.
 """
 import argparse
 import json
@@ -106,6 +108,26 @@ def bench_batched(attempts, warmup, ncols=256, graph=True):
                 ni=cases[0].net.ni, factor_ms=factor_ms)
 
 
+def bench_mixed(attempts, warmup, ncols=512):
+    """512 columns over the ni = 40 / 93 / 150 networks, nz uniform on [100, 150]."""
+    import numpy as np
+    from vulcan_amd import workloads as W
+    from vulcan_amd.device import MixedBatch
+    cases = W.mixed_cases(ncols)
+    mb = MixedBatch(cases, graph=True)
+    mb.integrate(warmup, use_conv=False)
+    c0 = mb.host_state()['count'].astype(np.int64).sum()
+    t0 = time.perf_counter()
+    mb.integrate(attempts, use_conv=False)
+    mb.synchronize()
+    dt = time.perf_counter() - t0
+    c1 = mb.host_state()['count'].astype(np.int64).sum()
+    groups = [dict(ni=int(b.ni), ncols=int(len(idx)), factor_variant_ms=[round(v, 3) for v in b.factor_ms])
+              for idx, b, _ in mb.groups]
+    mb.close()
+    return dict(time=dt, col_steps=int(c1 - c0), attempts=attempts, ncols=ncols, groups=groups)
+
+
 CPU_SNIPPET = r'''
 import json, sys, time
 sys.path.insert(0, %(root)r)
@@ -152,6 +174,8 @@ def main():
     ap.add_argument('--batched-attempts', type=int, default=40)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
+    ap.add_argument('--mixed-attempts', type=int, default=10)
+    ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
     ap.add_argument('--no-graph', action='store_true', help='eager launches instead of the captured HIP graph')
     args = ap.parse_args()
@@ -202,6 +226,12 @@ def main():
                           'factor_tflops': fl / (bt['factor_ms'] * 1e-3) / 1e12,
                           'factor_frac': fl / (bt['factor_ms'] * 1e-3) / FP64_PEAK,
                           'vs_single': (bt['col_steps'] / bt['time']) / value}
+    if not args.no_mixed:
+        print('[bench] mixed networks, 512 columns: %d attempts' % args.mixed_attempts, file=sys.stderr, flush=True)
+        mx = bench_mixed(args.mixed_attempts, 3)
+        out['mixed'] = {'workload': '512 columns over ni=40/93/150 networks (200/600/1100 pairs), nz uniform [100,150]',
+                        'value': mx['col_steps'] / mx['time'], 'unit': 'column-steps/s', 'attempts': mx['attempts'],
+                        'col_steps': mx['col_steps'], 'time_s': mx['time'], 'groups': mx['groups']}
     if not args.no_cpu:
         print('[bench] cpu baseline (%.0f s budget)' % args.cpu_budget, file=sys.stderr, flush=True)
         out['cpu_baseline'] = cpu_baseline(args.cpu_budget)

@@ -477,3 +477,30 @@ def test_graph_replay_is_bitwise_eager():
     for a_, b_ in zip(e, g):
         assert np.array_equal(np.asarray(a_).view(np.uint8), np.asarray(b_).view(np.uint8))
     assert e[3]['attempts'].min() == 6 * 7 + 8 or e[3]['done'].any()
+
+
+@gpu
+def test_mixed_networks_batch_matches_separate_batches():
+    """Columns of three networks (ni = 40 / 93 / 150) in one MixedBatch, each
+    network on its own stream, give exactly the results of one ColumnBatch
+    per network (same block-inversion variant)."""
+    torch = _torch()
+    from vulcan_amd.device import ColumnBatch, DevicePlan, MixedBatch
+    cases = W.mixed_cases(6)
+    for c in cases:
+        c.cfg.conv_step = 20
+    mb = MixedBatch(cases, history=21, graph=True, tune=False)
+    mb.integrate(12)
+    st = mb.host_state()
+    assert len(mb.groups) == 3
+    for idx, b, _ in mb.groups:
+        net = cases[int(idx[0])].net
+        ref = ColumnBatch(DevicePlan(net), [cases[i].col for i in idx], [cases[i].y0 for i in idx], cases[int(idx[0])].cfg,
+                          history=21, graph=False, tune=False)
+        ref.integrate(12)
+        torch.cuda.synchronize()
+        assert np.array_equal(ref.y.cpu().numpy(), b.y.cpu().numpy()), net.ni
+        rs = ref.host_state()
+        assert np.array_equal(rs['count'], st['count'][idx]) and np.array_equal(rs['t'], st['t'][idx])
+        ref.close()
+    mb.close()

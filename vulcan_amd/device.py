@@ -296,3 +296,55 @@ class ColumnBatch:
             self.close()
         except Exception:
             pass
+
+
+class MixedBatch:
+    """Columns over several networks (VULCAN runs one network per column;
+    a batch may mix them).  One DevicePlan and ColumnBatch per network, each
+    driven on its own default-priority stream so that the groups' attempts
+    run concurrently; the block-inversion kernel is tuned per network (per
+    ni).  `cases` carry `.net`, `.col`, `.y0`, `.cfg`."""
+
+    def __init__(self, cases, history: int = 0, graph: bool = True, tune: bool = True):
+        groups = {}
+        for idx, c in enumerate(cases):
+            groups.setdefault(id(c.net), []).append(idx)
+        self.ncols = len(cases)
+        self.groups = []
+        for idx in groups.values():
+            first = cases[idx[0]]
+            plan = DevicePlan(first.net)
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                b = ColumnBatch(plan, [cases[i].col for i in idx], [cases[i].y0 for i in idx], first.cfg,
+                                history=history, graph=graph, tune=tune)
+            self.groups.append((np.asarray(idx), b, stream))
+        torch.cuda.synchronize()
+
+    def integrate(self, n_attempts: int, use_conv=None, use_update_mu=True):
+        for _, b, stream in self.groups:
+            with torch.cuda.stream(stream):
+                b.integrate(n_attempts, use_conv=use_conv, use_update_mu=use_update_mu)
+
+    def synchronize(self):
+        torch.cuda.synchronize()
+
+    def host_state(self) -> np.ndarray:
+        torch.cuda.synchronize()
+        out = np.zeros(self.ncols, _lib.COLSTATE_DTYPE)
+        for idx, b, _ in self.groups:
+            out[idx] = b.host_state()
+        return out
+
+    def group_of(self, c: int):
+        """(batch, index within it) holding column c."""
+        for idx, b, _ in self.groups:
+            hit = np.nonzero(idx == c)[0]
+            if hit.size:
+                return b, int(hit[0])
+        raise IndexError(c)
+
+    def close(self):
+        for _, b, _ in self.groups:
+            b.close()
+        self.groups = []

@@ -1,83 +1,119 @@
 """Train the warm-start MLP on solver-generated steady states and compare
 time-to-steady-state of warm and cold starts on held-out columns.
 
-    python -m vulcan_amd.warmstart_study --train 4096 --test 512 --out ws.json
+    python -m vulcan_amd.warmstart_study generate --first 0 --count 2048 --out gen0.npz
+    python -m vulcan_amd.warmstart_study generate --first 2048 --count 2048 --out gen1.npz
+    python -m vulcan_amd.warmstart_study train-eval --data gen0.npz gen1.npz --test 512 --out report.json
 
-1. ``--train`` seeded columns (sampler in vulcan_amd/warmstart.py) are run
-   from their equilibrium cold start to VULCAN's steady state (``conv()``)
-   by the batched device integrator; converged columns become training
-   pairs (features -> log10 mixing ratios).
-2. The MLP (307 -> 4 x 1024 SiLU -> ni*nz) is trained on the device with
-   the library's BF16-MFMA kernels and Adam.
-3. ``--test`` held-out columns run twice to steady state: from the
-   equilibrium start and from the projected MLP prediction.  Reported:
-   median wall time to steady state per column (batched run, completion
-   observed at the host poll), accepted steps and attempts, and how many
-   columns met the convergence criteria.
+1. ``generate``: seeded columns (sampler in vulcan_amd/warmstart.py; column
+   i of the training stream is the same whatever the chunking) run from
+   their equilibrium cold start to VULCAN's steady state (``conv()``) in
+   batches on the device; the converged ones become training pairs
+   (features -> log10 mixing ratios, stored as float16).
+2. ``train-eval``: the MLP (307 -> 4 x 1024 SiLU -> ni*nz) is trained on
+   the device with the library's BF16-MFMA kernels and Adam; ``--test``
+   held-out columns (a different seed stream) then run twice to steady
+   state in one batch each: from the equilibrium start and from the
+   projected MLP prediction.  Reported: median wall time to steady state
+   per column (completion observed at the host poll), accepted steps and
+   attempts, convergence counts and the agreement of the two steady states.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import time
 
 import numpy as np
 import torch
 
 from . import warmstart as WS
-from .config import Config
 from .mlp import MLP
-from .workloads import full_network
+from .workloads import _pool_map, full_network
 
 
 def log(msg):
     print('[warmstart] ' + msg, flush=True)
 
 
-def generate(net, specs, batch, max_attempts, poll, plan):
-    feats, targs, res_all = [], [], []
-    for b0 in range(0, len(specs), batch):
+def _cold_case(args):
+    net, spec = args
+    return WS.make_case(net, spec)
+
+
+def _warm_case(args):
+    net, spec, logx = args
+    xp = 10.0 ** logx.reshape(len(spec.T), net.ni).astype(np.float64)
+    xp = xp / xp.sum(axis=1, keepdims=True)
+    xw = WS.project(net, xp, WS.element_vector(net, spec.logab))
+    return WS.make_case(net, spec, ymix0=xw)
+
+
+def _cases(fn, net, items):
+    out = _pool_map(fn, [(net,) + tuple(it) for it in items])
+    for c in out:
+        c.net = net
+    return out
+
+
+def generate(first, count, seed, nz, batch, max_attempts, poll, out):
+    net = full_network()
+    from .device import DevicePlan
+    plan = DevicePlan(net)
+    specs = WS.sample_specs(first + count, seed, nz)[first:]
+    X = WS.features(specs).astype(np.float32)
+    Y = np.zeros((count, nz * net.ni), np.float16)
+    rec = {k: np.zeros(count, t) for k, t in (('done', bool), ('count', np.int32), ('attempts', np.int32),
+                                             ('wall', np.float32), ('t_model', np.float64))}
+    t_all = time.perf_counter()
+    for b0 in range(0, count, batch):
         chunk = specs[b0:b0 + batch]
-        cases = [WS.make_case(net, s) for s in chunk]
         t0 = time.perf_counter()
+        cases = _cases(_cold_case, net, [(s,) for s in chunk])
+        log('batch %d-%d: %d cold starts built in %.1f s' % (first + b0, first + b0 + len(chunk), len(chunk),
+                                                             time.perf_counter() - t0))
         res = WS.run_to_steady(net, cases, max_attempts=max_attempts, poll=poll, plan=plan, log=log)
+        sl = slice(b0, b0 + len(chunk))
+        rec['done'][sl] = res.done
+        rec['count'][sl] = res.count
+        rec['attempts'][sl] = res.attempts
+        rec['wall'][sl] = res.wall
+        rec['t_model'][sl] = res.t_model
+        Y[sl] = np.stack([WS.targets(ym[None])[0] for ym in res.ymix]).astype(np.float16)
         log('batch %d-%d: %d/%d converged, median steps %d, %.1f s' % (
-            b0, b0 + len(chunk), int(res.done.sum()), len(chunk), int(np.median(res.count)),
+            first + b0, first + b0 + len(chunk), int(res.done.sum()), len(chunk), int(np.median(res.count)),
             time.perf_counter() - t0))
-        X = WS.features(chunk)
-        Y = np.stack([WS.targets(ym[None])[0] for ym in res.ymix])
-        feats.append(X[res.done])
-        targs.append(Y[res.done])
-        res_all.append(res)
-    return np.concatenate(feats), np.concatenate(targs), res_all
+    np.savez_compressed(out, index=np.arange(first, first + count, dtype=np.int32), X=X, Y=Y, seed=seed, nz=nz,
+                        **rec)
+    log('generated %d columns (%d converged) in %.1f s -> %s' % (count, int(rec['done'].sum()),
+                                                                time.perf_counter() - t_all, out))
 
 
-def train(net_sizes, X, Y, epochs, batch, lr, seed=0):
+def train(sizes, X, Y, epochs, batch, lr, seed=0):
     dev = torch.device('cuda')
     xm, xs = X.mean(0), X.std(0) + 1e-6
     ym, ys = Y.mean(0), Y.std(0) + 1e-3
     Xn = torch.as_tensor(((X - xm) / xs).astype(np.float32), device=dev)
     Yn = torch.as_tensor(((Y - ym) / ys).astype(np.float32), device=dev)
-    mlp = MLP(net_sizes, seed=seed)
+    mlp = MLP(sizes, seed=seed)
     n = X.shape[0]
+    batch = min(batch, n)
     rng = np.random.default_rng(seed)
-    steps = 0
-    total = epochs * max(1, n // batch)
+    steps, total = 0, epochs * max(1, n // batch)
+    losses = []
     for ep in range(epochs):
         perm = rng.permutation(n)
         losses = []
         for b0 in range(0, n - batch + 1, batch):
             idx = torch.as_tensor(perm[b0:b0 + batch], device=dev)
-            frac = steps / max(total, 1)
-            lr_t = lr * 0.5 * (1 + np.cos(np.pi * frac))
+            lr_t = lr * 0.5 * (1 + np.cos(np.pi * steps / max(total, 1)))
             loss, _ = mlp.loss_and_backward(Xn[idx].contiguous(), Yn[idx].contiguous())
             mlp.adam_step(lr=lr_t, wd=1e-5)
             losses.append(loss)
             steps += 1
         if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
             log('epoch %d: loss %.4e' % (ep, float(np.mean(losses))))
-    return mlp, (xm, xs, ym, ys)
+    return mlp, (xm, xs, ym, ys), float(np.mean(losses))
 
 
 def predict(mlp, norm, X):
@@ -89,82 +125,107 @@ def predict(mlp, norm, X):
     return np.concatenate(out) * ys + ym
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument('--train', type=int, default=4096)
-    ap.add_argument('--test', type=int, default=512)
-    ap.add_argument('--batch', type=int, default=512)
-    ap.add_argument('--max-attempts', type=int, default=12000)
-    ap.add_argument('--poll', type=int, default=16)
-    ap.add_argument('--epochs', type=int, default=300)
-    ap.add_argument('--mb', type=int, default=256)
-    ap.add_argument('--lr', type=float, default=1e-3)
-    ap.add_argument('--seed', type=int, default=5005)
-    ap.add_argument('--out', default='warmstart.json')
-    ap.add_argument('--save-model', default=None)
-    a = ap.parse_args(argv)
-    torch.cuda.synchronize()
-    net = full_network()
-    from .device import DevicePlan
-    plan = DevicePlan(net)
-    log('generating %d training steady states' % a.train)
-    t0 = time.perf_counter()
-    specs = WS.sample_specs(a.train, a.seed)
-    X, Y, _ = generate(net, specs, a.batch, a.max_attempts, a.poll, plan)
-    t_gen = time.perf_counter() - t0
-    log('training on %d converged columns' % X.shape[0])
-    sizes = [X.shape[1], 1024, 1024, 1024, 1024, Y.shape[1]]
-    t1 = time.perf_counter()
-    mlp, norm = train(sizes, X, Y, a.epochs, a.mb, a.lr)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t1
-    if a.save_model:
-        np.savez_compressed(a.save_model, **{'W%d' % i: w for i, w in enumerate(mlp.state_dict()['W'])},
-                            **{'b%d' % i: b for i, b in enumerate(mlp.state_dict()['b'])},
-                            xm=norm[0], xs=norm[1], ym=norm[2], ys=norm[3], sizes=np.array(sizes))
-    test = WS.sample_specs(a.test, a.seed + 1)
-    Xt = WS.features(test)
-    t2 = time.perf_counter()
-    P = predict(mlp, norm, Xt)
-    t_pred = time.perf_counter() - t2
-    cold = [WS.make_case(net, s) for s in test]
-    warm = []
-    for s, p in zip(test, P):
-        xp = 10.0 ** p.reshape(WS.NZ, net.ni)
-        xp = xp / xp.sum(axis=1, keepdims=True)
-        xw = WS.project(net, xp, WS.element_vector(net, s.logab))
-        warm.append(WS.make_case(net, s, ymix0=xw))
-    log('held-out cold runs')
-    rc = WS.run_to_steady(net, cold, max_attempts=a.max_attempts, poll=a.poll, plan=plan, log=log)
-    log('held-out warm runs')
-    rw = WS.run_to_steady(net, warm, max_attempts=a.max_attempts, poll=a.poll, plan=plan, log=log)
+def compare(net, specs, P, max_attempts, poll, plan=None):
+    """Cold (equilibrium) and warm (projected prediction P, log10 mixing
+    ratios) runs of the same columns to steady state, one batch each."""
+    cold = _cases(_cold_case, net, [(s,) for s in specs])
+    warm = _cases(_warm_case, net, [(s, p) for s, p in zip(specs, P)])
+    log('cold runs (%d columns)' % len(specs))
+    rc = WS.run_to_steady(net, cold, max_attempts=max_attempts, poll=poll, plan=plan, log=log)
+    log('warm runs (%d columns)' % len(specs))
+    rw = WS.run_to_steady(net, warm, max_attempts=max_attempts, poll=poll, plan=plan, log=log)
     both = rc.done & rw.done
-    # agreement of the two steady states (same criteria, same element inventory)
     dev = []
     for c in np.nonzero(both)[0]:
         lc = np.log10(np.maximum(rc.ymix[c], 1e-30))
         lw = np.log10(np.maximum(rw.ymix[c], 1e-30))
         sig = rc.ymix[c] > 1e-12
         dev.append(float(np.max(np.abs(lc - lw)[sig])))
+    med = lambda a: float(np.median(a[both])) if both.any() else None
     rep = {
-        'train_columns': a.train, 'train_converged': int(X.shape[0]), 'gen_seconds': t_gen,
-        'train_seconds': t_train, 'predict_seconds_512': t_pred, 'epochs': a.epochs,
-        'test_columns': a.test, 'cold_converged': int(rc.done.sum()), 'warm_converged': int(rw.done.sum()),
-        'median_wall_cold_s': float(np.nanmedian(rc.wall[both])) if both.any() else None,
-        'median_wall_warm_s': float(np.nanmedian(rw.wall[both])) if both.any() else None,
-        'median_steps_cold': float(np.median(rc.count[both])) if both.any() else None,
-        'median_steps_warm': float(np.median(rw.count[both])) if both.any() else None,
-        'median_attempts_cold': float(np.median(rc.attempts[both])) if both.any() else None,
-        'median_attempts_warm': float(np.median(rw.attempts[both])) if both.any() else None,
+        'test_columns': len(specs), 'cold_converged': int(rc.done.sum()), 'warm_converged': int(rw.done.sum()),
+        'median_wall_cold_s': med(rc.wall), 'median_wall_warm_s': med(rw.wall),
+        'median_steps_cold': med(rc.count), 'median_steps_warm': med(rw.count),
+        'median_attempts_cold': med(rc.attempts), 'median_attempts_warm': med(rw.attempts),
+        'batch_wall_cold_s': float(np.nanmax(rc.wall)) if rc.done.any() else None,
+        'batch_wall_warm_s': float(np.nanmax(rw.wall)) if rw.done.any() else None,
         'median_max_log10_diff_steady_states': float(np.median(dev)) if dev else None,
     }
     if rep['median_wall_cold_s'] and rep['median_wall_warm_s']:
         rep['speedup_median_wall'] = rep['median_wall_cold_s'] / rep['median_wall_warm_s']
         rep['speedup_median_attempts'] = rep['median_attempts_cold'] / rep['median_attempts_warm']
+    return rep
+
+
+def train_eval(data, test, test_seed, hidden, layers, epochs, mb, lr, max_attempts, poll, out, save_model=None):
+    net = full_network()
+    parts = [np.load(f) for f in data]
+    nz = int(parts[0]['nz'])
+    X = np.concatenate([p['X'][p['done']] for p in parts]).astype(np.float64)
+    Y = np.concatenate([p['Y'][p['done']] for p in parts]).astype(np.float64)
+    n_gen = int(sum(len(p['done']) for p in parts))
+    gen_attempts = np.concatenate([p['attempts'][p['done']] for p in parts])
+    log('training on %d converged columns of %d generated' % (X.shape[0], n_gen))
+    sizes = [X.shape[1]] + [hidden] * layers + [Y.shape[1]]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mlp, norm, final_loss = train(sizes, X, Y, epochs, mb, lr)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    if save_model:
+        sd = mlp.state_dict()
+        np.savez_compressed(save_model, **{'W%d' % i: w for i, w in enumerate(sd['W'])},
+                            **{'b%d' % i: b for i, b in enumerate(sd['b'])},
+                            xm=norm[0], xs=norm[1], ym=norm[2], ys=norm[3], sizes=np.array(sizes))
+    specs = WS.sample_specs(test, test_seed, nz)
+    Xt = WS.features(specs)
+    t2 = time.perf_counter()
+    P = predict(mlp, norm, Xt)
+    t_pred = time.perf_counter() - t2
+    from .device import DevicePlan
+    rep = compare(net, specs, P, max_attempts, poll, plan=DevicePlan(net))
+    rep.update({'train_columns_generated': n_gen, 'train_columns_converged': int(X.shape[0]),
+                'train_median_attempts_cold': float(np.median(gen_attempts)) if len(gen_attempts) else None,
+                'mlp_sizes': sizes, 'epochs': epochs, 'final_loss_normalised': final_loss,
+                'train_seconds': t_train, 'predict_seconds': t_pred, 'nz': nz, 'ni': net.ni,
+                'test_seed': test_seed})
     log(json.dumps(rep))
-    with open(a.out, 'w') as f:
+    with open(out, 'w') as f:
         json.dump(rep, f, indent=1)
     return rep
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    g = sub.add_parser('generate')
+    g.add_argument('--first', type=int, default=0)
+    g.add_argument('--count', type=int, default=4096)
+    g.add_argument('--seed', type=int, default=5005)
+    g.add_argument('--nz', type=int, default=WS.NZ)
+    g.add_argument('--batch', type=int, default=1024)
+    g.add_argument('--max-attempts', type=int, default=8000)
+    g.add_argument('--poll', type=int, default=16)
+    g.add_argument('--out', required=True)
+    t = sub.add_parser('train-eval')
+    t.add_argument('--data', nargs='+', required=True)
+    t.add_argument('--test', type=int, default=512)
+    t.add_argument('--test-seed', type=int, default=6006)
+    t.add_argument('--hidden', type=int, default=1024)
+    t.add_argument('--layers', type=int, default=4)
+    t.add_argument('--epochs', type=int, default=300)
+    t.add_argument('--mb', type=int, default=256)
+    t.add_argument('--lr', type=float, default=1e-3)
+    t.add_argument('--max-attempts', type=int, default=8000)
+    t.add_argument('--poll', type=int, default=16)
+    t.add_argument('--out', required=True)
+    t.add_argument('--save-model', default=None)
+    a = ap.parse_args(argv)
+    if a.cmd == 'generate':
+        generate(a.first, a.count, a.seed, a.nz, a.batch, a.max_attempts, a.poll, a.out)
+    else:
+        train_eval(a.data, a.test, a.test_seed, a.hidden, a.layers, a.epochs, a.mb, a.lr, a.max_attempts, a.poll,
+                   a.out, a.save_model)
 
 
 if __name__ == '__main__':

@@ -24,6 +24,7 @@ elements at the S/H value).  ``y = n_0 * ymix``.
 from __future__ import annotations
 
 import dataclasses
+import os
 from typing import List, Optional
 
 import numpy as np
@@ -104,32 +105,53 @@ def iso_gradient_profile(nz: int, T_top: float, T_bot: float, P_b=1e9, P_t=1e4, 
     return T_top + (T_bot - T_top) * f
 
 
+def _pool_map(fn, items, workers=None):
+    """Map over case descriptions, in worker processes (spawned: no state is
+    inherited from the caller, which may hold a GPU context) when there are
+    many; results come back in order."""
+    workers = workers or min(16, os.cpu_count() or 1, len(items))
+    if workers <= 1 or len(items) < 16:
+        return [fn(it) for it in items]
+    import multiprocessing as mp
+    with mp.get_context('spawn').Pool(workers) as pool:
+        return pool.map(fn, items, chunksize=max(1, len(items) // (4 * workers)))
+
+
+def _batched_one(args):
+    s, nz_range, net = args
+    rng = np.random.default_rng(s)
+    nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
+    T_top, T_bot = rng.uniform(400., 2500., 2)
+    K = 10.0 ** rng.uniform(5, 11)
+    prof = iso_gradient_profile(nz, T_top, T_bot)
+    return _case(net, nz, T_bot, T_top, K, None, rng, Config(), tau=True, T_profile=prof)
+
+
 def batched_cases(ncols: int = 256, net: Optional[Network] = None, nz_range=(80, 200),
                   seed: int = SEED_BATCH) -> List[ColumnCase]:
     net = net or full_network()
-    cfg = Config()
-    out = []
-    for s in np.random.SeedSequence(seed).spawn(ncols):
-        rng = np.random.default_rng(s)
-        nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
-        T_top, T_bot = rng.uniform(400., 2500., 2)
-        K = 10.0 ** rng.uniform(5, 11)
-        prof = iso_gradient_profile(nz, T_top, T_bot)
-        out.append(_case(net, nz, T_bot, T_top, K, None, rng, cfg, tau=True, T_profile=prof))
+    out = _pool_map(_batched_one, [(s, nz_range, net) for s in np.random.SeedSequence(seed).spawn(ncols)])
+    for c in out:
+        c.net = net
     return out
+
+
+def _mixed_one(args):
+    s, net = args
+    rng = np.random.default_rng(s)
+    nz = int(rng.integers(100, 151))
+    T_top, T_bot = rng.uniform(400., 2500., 2)
+    K = 10.0 ** rng.uniform(5, 11)
+    prof = iso_gradient_profile(nz, T_top, T_bot)
+    return _case(net, nz, T_bot, T_top, K, None, rng, Config(), tau=False, T_profile=prof)
 
 
 def mixed_cases(ncols: int = 512, nets: Optional[List[Network]] = None,
                 seed: int = SEED_MIXED) -> List[ColumnCase]:
+    """Column c uses network c % 3."""
     nets = nets or mixed_networks()
-    cfg = Config()
-    out = []
-    for c, s in enumerate(np.random.SeedSequence(seed).spawn(ncols)):
-        rng = np.random.default_rng(s)
-        net = nets[c % len(nets)]
-        nz = int(rng.integers(100, 151))
-        T_top, T_bot = rng.uniform(400., 2500., 2)
-        K = 10.0 ** rng.uniform(5, 11)
-        prof = iso_gradient_profile(nz, T_top, T_bot)
-        out.append(_case(net, nz, T_bot, T_top, K, None, rng, cfg, tau=False, T_profile=prof))
+    seeds = np.random.SeedSequence(seed).spawn(ncols)
+    out = _pool_map(_mixed_one, [(s, nets[c % len(nets)]) for c, s in enumerate(seeds)])
+    for c, case in enumerate(out):
+        case.net = nets[c % len(nets)]
     return out
