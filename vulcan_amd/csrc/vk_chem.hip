@@ -164,6 +164,20 @@ __device__ inline double dside(int4 t4, double kcol, const double *yl, double m,
     return tot;
 }
 
+// ysum[l] = sum_i y[l][i] in numpy's pairwise order, one thread per layer
+__global__ void k_ysum(int L, int ni, const double *__restrict__ y, double *__restrict__ ysum) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < L) ysum[l] = pw_sum(y + (size_t)l * ni, ni);
+}
+
+// the three layer totals a layer's diffusion terms use (0 outside the column)
+__device__ inline void ysum_around(const double *ysum, const LayerCtx &x, double &ym, double &y0, double &yp) {
+    const double *ys = ysum + x.off;
+    ym = x.j > 0 ? ys[x.j - 1] : 0.0;
+    y0 = ys[x.j];
+    yp = x.j < x.nz - 1 ? ys[x.j + 1] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // chemdf + diffdf (+ stage-2 term), one workgroup per layer
 // mode bit 0: include chemistry, bit 1: include diffusion
@@ -172,27 +186,21 @@ __device__ inline double dside(int4 t4, double kcol, const double *yl, double m,
 __global__ void __launch_bounds__(BLK) k_rhs(
     int ni, int npairs, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ chem_ptr,
-    const int *__restrict__ chem_pair, const int *__restrict__ chem_sign, const double *__restrict__ ms,
+    const int2 *__restrict__ chem_ent, const double *__restrict__ ms,
     const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
     const double *__restrict__ M, int mode, int mol, const double *__restrict__ k1,
-    const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ out) {
+    const double *__restrict__ dtc, const double *__restrict__ ysum, const vk_colstate *__restrict__ st,
+    double *__restrict__ out) {
     extern __shared__ double smem[];
     double *yl = smem;              // ni
     double *v = smem + VK_NI_MAX;   // npairs
-    __shared__ double ysum3[3];
     const int l = blockIdx.x;
     LayerCtx x = layer_ctx(off, layer_col, l);
     if (st && st[x.c].done) return;
     const double *yrow = y + (size_t)l * ni;
     for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
-    if (mode & 2) {
-        if (threadIdx.x < 3) {
-            int jj = x.j - 1 + threadIdx.x;
-            double s = 0.0;
-            if (jj >= 0 && jj < x.nz) s = pw_sum(y + (size_t)(x.off + jj) * ni, ni);
-            ysum3[threadIdx.x] = s;
-        }
-    }
+    double ysm = 0.0, ys0 = 0.0, ysp = 0.0;
+    if (mode & 2) ysum_around(ysum, x, ysm, ys0, ysp);
     __syncthreads();
     if (mode & 1) {
         const double *kl = k + (size_t)l * 2 * npairs;
@@ -207,17 +215,21 @@ __global__ void __launch_bounds__(BLK) k_rhs(
     for (int i = threadIdx.x; i < ni; i += blockDim.x) {
         double chem = 0.0;
         if (mode & 1) {
-            int lo = chem_ptr[i], hi = chem_ptr[i + 1];
+            const int lo = chem_ptr[i], hi = chem_ptr[i + 1];
             if (lo < hi) {
-                chem = chem_sign[lo] * v[chem_pair[lo]];
-                for (int e = lo + 1; e < hi; e++) chem = chem + chem_sign[e] * v[chem_pair[e]];
+                int2 t = chem_ent[lo];                 // (pair, sign)
+                chem = t.y * v[t.x];
+                for (int e = lo + 1; e < hi; e++) {
+                    t = chem_ent[e];
+                    chem = chem + t.y * v[t.x];
+                }
             }
         }
         double val = chem;
         if (mode & 2) {
             double ym = x.j > 0 ? y[(size_t)(l - 1) * ni + i] : 0.0;
             double yp = x.j < x.nz - 1 ? y[(size_t)(l + 1) * ni + i] : 0.0;
-            double d = diff_value(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ym, yl[i], yp, ms[i], alpha[i], mol != 0);
+            double d = diff_value(ca, x.j, i, ysm, ys0, ysp, ym, yl[i], yp, ms[i], alpha[i], mol != 0);
             val = (mode & 1) ? chem + d : d;
         }
         if (k1) val = val - c2 * k1[(size_t)l * ni + i];
@@ -231,30 +243,27 @@ __global__ void __launch_bounds__(BLK) k_rhs(
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(BLK) k_jac_lhs(
-    int ni, int npairs, int nslot, const int *__restrict__ off, const int *__restrict__ layer_col,
+    int ni, int npairs, int nslot, int nent, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
-    const int *__restrict__ slot_sp, const int2 *__restrict__ jac_rng, const vk_jac_term *__restrict__ jac_term,
-    const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm,
-    const double *__restrict__ y, const double *__restrict__ k, const double *__restrict__ M, int mode, int mol,
-    const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ D,
+    const int *__restrict__ slot_sp, const int4 *__restrict__ jac_ent, const vk_jac_term *__restrict__ jac_term,
+    const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y,
+    const double *__restrict__ k, const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
+    const double *__restrict__ ysum, const vk_colstate *__restrict__ st, double *__restrict__ D,
     double *__restrict__ Lsub, double *__restrict__ Usup) {
     extern __shared__ double smem[];
     double *yl = smem;                       // ni
     double *dg0 = smem + VK_NI_MAX;          // diffusion terms of the diagonal, per species
     double *dg1 = smem + 2 * VK_NI_MAX;
     double *dv = smem + 3 * VK_NI_MAX;       // nslot
-    __shared__ double ysum3[3];
     const int l = blockIdx.x;
     LayerCtx x = layer_ctx(off, layer_col, l);
     if (st && st[x.c].done) return;
     const double *yrow = y + (size_t)l * ni;
     for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
-    if (mode == 1 && threadIdx.x < 3) {
-        int jj = x.j - 1 + threadIdx.x;
-        double s = 0.0;
-        if (jj >= 0 && jj < x.nz) s = pw_sum(y + (size_t)(x.off + jj) * ni, ni);
-        ysum3[threadIdx.x] = s;
-    }
+    const int nn = ni * ni;
+    double *Dl = D + (size_t)l * nn;
+    // entries without a chemical term are zero (the diagonal is rewritten below)
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) Dl[e] = 0.0;
     __syncthreads();
     const double *kl = k + (size_t)l * 2 * npairs;
     const double m = M[l];
@@ -269,9 +278,11 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     if (mode == 1) {
         // diffusion part of the lhs (op.py:2012-2040): once per species
         c0 = 1. / (R_ROS * dtc[x.c]);
+        double ysm, ys0, ysp;
+        ysum_around(ysum, x, ysm, ys0, ysp);
         ColAtm ca = col_atm(atm, x.off, x.nz, ni);
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
-            LhsDiff r = lhs_diff(ca, x.j, i, ysum3[0], ysum3[1], ysum3[2], ms[i], alpha[i], mol != 0);
+            LhsDiff r = lhs_diff(ca, x.j, i, ysm, ys0, ysp, ms[i], alpha[i], mol != 0);
             dg0[i] = r.d0;
             dg1[i] = r.d1;
             double u = 0.0, lo = 0.0;
@@ -282,31 +293,30 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         }
     }
     __syncthreads();
-    const int nn = ni * ni;
-    double *Dl = D + (size_t)l * nn;
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-        const int2 rg = jac_rng[e];
+    // the entries with chemical terms, and every diagonal entry
+    for (int q = threadIdx.x; q < nent; q += blockDim.x) {
+        const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
         double J = 0.0;
-        if (rg.y > rg.x) {
-            vk_jac_term t = jac_term[rg.x];
+        if (en.z > en.y) {
+            vk_jac_term t = jac_term[en.y];
             J = t.coef * dv[t.slot];
-            for (int q = rg.x + 1; q < rg.y; q++) {
-                t = jac_term[q];
+            for (int u = en.y + 1; u < en.z; u++) {
+                t = jac_term[u];
                 J = J + t.coef * dv[t.slot];
             }
         }
         if (mode == 0) {
-            Dl[e] = J;
+            Dl[en.x] = J;
             continue;
         }
-        const int i = e / ni, mm = e - i * ni;
+        const int i = en.w;
         double val = -J;
-        if (i == mm) {
+        if (en.x == i * ni + i) {
             val = c0 + val;
             val = val - dg0[i];
             if (mol) val = val - dg1[i];
         }
-        Dl[e] = val;
+        Dl[en.x] = val;
     }
 }
 
@@ -336,55 +346,86 @@ extern "C" int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_at
     return 0;
 }
 
+static int launch_ysum(const vk_columns *cols, int ni, const double *y, double *ysum, hipStream_t s) {
+    hipLaunchKernelGGL(k_ysum, dim3((cols->L + 127) / 128), dim3(128), 0, s, cols->L, ni, y, ysum);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ysum: scratch of L doubles (needed when the diffusion terms are evaluated)
 int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
                   const double *k, int mode, int mol, const double *k1, const double *dtc,
-                  const vk_colstate *st, double *out, hipStream_t s) {
+                  const vk_colstate *st, double *ysum, double *out, hipStream_t s) {
     vk_atm a = *atm;
+    if (mode & 2) {
+        if (!ysum) return VK_EINVAL;
+        int r = launch_ysum(cols, plan->ni, y, ysum, s);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, cols->d_off,
-                       cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_pair, plan->chem_sign,
-                       plan->ms, plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, st, out);
+                       cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_ent, plan->ms,
+                       plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, ysum, st, out);
     VK_CHECK(hipGetLastError());
     return 0;
 }
 
 int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
                       const double *k, const double *M, int mode, int mol, const double *dtc,
-                      const vk_colstate *st, double *D, double *Lsub, double *Usup, hipStream_t s) {
+                      const vk_colstate *st, double *ysum, double *D, double *Lsub, double *Usup, hipStream_t s) {
     vk_atm a{};
     if (atm) a = *atm;
+    if (mode == 1) {
+        if (!ysum) return VK_EINVAL;
+        int r = launch_ysum(cols, plan->ni, y, ysum, s);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
-                       cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair, plan->slot_sp,
-                       plan->jac_rng, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
-                       dtc, st, D, Lsub, Usup);
+                       plan->njent, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
+                       plan->slot_sp, plan->jac_ent, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
+                       dtc, ysum, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;
 }
+
+// temporary layer totals for the one-shot entry points
+struct YsumTmp {
+    double *p = nullptr;
+    hipStream_t s;
+    explicit YsumTmp(hipStream_t st) : s(st) {}
+    int alloc(int L) { return hipMallocAsync((void **)&p, sizeof(double) * (size_t)L, s) == hipSuccess ? 0 : VK_ENOMEM; }
+    ~YsumTmp() { if (p) (void)hipFreeAsync(p, s); }
+};
 
 extern "C" int vk_chemdf(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
                          const double *d_k, double *d_dydt, void *stream) {
     if (!plan || !cols || !d_y || !d_M || !d_k || !d_dydt) return VK_EINVAL;
     vk_atm a{};
     a.M = d_M;
-    return vk_launch_rhs(plan, cols, &a, d_y, d_k, 1, 0, nullptr, nullptr, nullptr, d_dydt, S(stream));
+    return vk_launch_rhs(plan, cols, &a, d_y, d_k, 1, 0, nullptr, nullptr, nullptr, nullptr, d_dydt, S(stream));
 }
 
 extern "C" int vk_diffdf(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
                          int use_moldiff, double *d_out, void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_out) return VK_EINVAL;
-    return vk_launch_rhs(plan, cols, atm, d_y, nullptr, 2, use_moldiff, nullptr, nullptr, nullptr, d_out, S(stream));
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_rhs(plan, cols, atm, d_y, nullptr, 2, use_moldiff, nullptr, nullptr, nullptr, ys.p, d_out,
+                         S(stream));
 }
 
 extern "C" int vk_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
                       const double *d_k, int use_moldiff, const double *d_k1, const double *d_dt, double *d_out,
                       void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_k || !d_out || (d_k1 && !d_dt)) return VK_EINVAL;
-    return vk_launch_rhs(plan, cols, atm, d_y, d_k, 3, use_moldiff, d_k1, d_dt, nullptr, d_out, S(stream));
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_rhs(plan, cols, atm, d_y, d_k, 3, use_moldiff, d_k1, d_dt, nullptr, ys.p, d_out, S(stream));
 }
 
 extern "C" int vk_jac(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
                       const double *d_k, double *d_J, void *stream) {
     if (!plan || !cols || !d_y || !d_M || !d_k || !d_J) return VK_EINVAL;
-    return vk_launch_jac_lhs(plan, cols, nullptr, d_y, d_k, d_M, 0, 0, nullptr, nullptr, d_J, nullptr, nullptr,
+    return vk_launch_jac_lhs(plan, cols, nullptr, d_y, d_k, d_M, 0, 0, nullptr, nullptr, nullptr, d_J, nullptr, nullptr,
                              S(stream));
 }
 
@@ -392,6 +433,8 @@ extern "C" int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
                       const double *d_k, int use_moldiff, const double *d_dt, double *d_D, double *d_Lsub,
                       double *d_Usup, void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_k || !d_dt || !d_D || !d_Lsub || !d_Usup) return VK_EINVAL;
-    return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, d_D, d_Lsub, d_Usup,
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, ys.p, d_D, d_Lsub, d_Usup,
                              S(stream));
 }

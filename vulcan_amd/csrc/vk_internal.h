@@ -26,9 +26,11 @@ struct vk_plan {
     int *reac, *prod, *kind, *rev;
     double *par, *nasa9, *atom_mat, *ms, *alpha;
     int *chem_ptr, *chem_pair, *chem_sign;
+    int2 *chem_ent;            // (chem_pair, chem_sign) interleaved
     int *slot_pair, *slot_sp;
     int *jac_map, *jac_ptr, *jac_slot;
-    int2 *jac_rng;             // per dense entry: [jac_ptr[jm], jac_ptr[jm+1]) or empty
+    int njent;                 // entries with chemical terms plus the diagonal
+    int4 *jac_ent;             // (dense index, first term, end of terms, row), by dense index
     vk_jac_term *jac_term;     // jac_coef / jac_slot interleaved
     double *jac_coef;
     std::vector<void *> allocs;
@@ -53,6 +55,7 @@ struct vk_integrator {
     double *b1, *x1, *b2, *x2, *work;   // right-hand sides and solutions of the two stages
     double *ztmp;                       // forward-sweep scratch of the twisted solve
     double *yk2, *part;
+    double *ysum;                       // layer totals for the diffusion terms
     int *info;
     double *yhist;             // [history][L][ni]
     double *thist;             // [ncols][tcap]

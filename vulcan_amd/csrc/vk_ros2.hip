@@ -10,10 +10,10 @@
 using namespace vk;
 
 int vk_launch_rhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *, int, int,
-                  const double *, const double *, const vk_colstate *, double *, hipStream_t);
+                  const double *, const double *, const vk_colstate *, double *, double *, hipStream_t);
 int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *,
                       const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
-                      hipStream_t);
+                      double *, hipStream_t);
 int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
                      int *, const vk_colstate *, hipStream_t);
 int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
@@ -438,12 +438,19 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
     r = r ? r : upload(v, &p->jac_slot, n->jac_slot, (size_t)n->njac);
     r = r ? r : upload(v, &p->jac_coef, n->jac_coef, (size_t)n->njac);
     if (!r) {
-        std::vector<int2> rng((size_t)n->ni * n->ni);
-        for (size_t e = 0; e < rng.size(); e++) {
-            const int jm = n->jac_map[e];
-            rng[e] = jm >= 0 ? make_int2(n->jac_ptr[jm], n->jac_ptr[jm + 1]) : make_int2(0, 0);
+        std::vector<int4> ent;
+        for (int e = 0; e < n->ni * n->ni; e++) {
+            const int jm = n->jac_map[e], row = e / n->ni;
+            if (jm >= 0) ent.push_back(make_int4(e, n->jac_ptr[jm], n->jac_ptr[jm + 1], row));
+            else if (e % n->ni == row) ent.push_back(make_int4(e, 0, 0, row));
         }
-        r = upload(v, &p->jac_rng, rng.data(), rng.size());
+        p->njent = (int)ent.size();
+        r = upload(v, &p->jac_ent, ent.data(), ent.size());
+    }
+    if (!r) {
+        std::vector<int2> ce((size_t)n->nchem);
+        for (size_t q = 0; q < ce.size(); q++) ce[q] = make_int2(n->chem_pair[q], n->chem_sign[q]);
+        r = upload(v, &p->chem_ent, ce.data(), ce.size());
     }
     if (!r) {
         std::vector<vk_jac_term> terms((size_t)n->njac);
@@ -535,6 +542,7 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
     r = r ? r : dalloc(v, &it->ztmp, L * ni);
     r = r ? r : dalloc(v, &it->yk2, L * ni);
     r = r ? r : dalloc(v, &it->part, L * NP);
+    r = r ? r : dalloc(v, &it->ysum, L);
     r = r ? r : dalloc(v, &it->info, nc);
     r = r ? r : dalloc(v, &it->flags, nc);
     r = r ? r : dalloc(v, &it->colscr, nc * 16);
@@ -610,9 +618,10 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     // dt per column lives in the state struct; gather it contiguously
     double *dt = it->colscr;
     hipLaunchKernelGGL(k_gather_dt, dim3((cl->ncols + 127) / 128), dim3(128), 0, s, cl->ncols, st, dt);
-    int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->b1, s);
+    int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->ysum, it->b1, s);
     if (r) return r;
-    r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->D, it->Lsub, it->Usup, s);
+    r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->ysum, it->D, it->Lsub, it->Usup,
+                          s);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
     r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, s);
@@ -623,7 +632,8 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     if (r) return r;
     const int n = (int)(L * ni);
     hipLaunchKernelGGL(k_yk2, dim3((n + 255) / 256), dim3(256), 0, s, n, ni, cl->d_layer_col, y, it->x1, st, it->yk2);
-    r = vk_launch_rhs(p, cl, atm, it->yk2, k, 3, prm->use_moldiff, it->x1, dt, st, it->b2, s);   // f(yk2) - 2/(r h) k1
+    r = vk_launch_rhs(p, cl, atm, it->yk2, k, 3, prm->use_moldiff, it->x1, dt, st, it->ysum, it->b2,
+                      s);   // f(yk2) - 2/(r h) k1
     if (r) return r;
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b2, it->x2, it->work, it->ztmp, prm->refine,
                                 st, s);   // k2

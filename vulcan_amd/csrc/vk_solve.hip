@@ -235,6 +235,7 @@ struct BlkShared {
     static constexpr int N16 = 16 * NB;
     double pan[2][16][N16 + 1];   // panel columns, [k][row], double-buffered by panel parity
     double rb[16][N16];           // old pivot rows of the panel, [s][col]
+    double prow[16];              // current pivot row of the panel factorisation
     int ipos[N16];
     int perm[N16];
     int info;
@@ -316,30 +317,38 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         const unsigned bhi = (unsigned)(best >> 32);
         const int p = (blo | bhi) ? 255 - (int)(blo & 0xffu) : 0;
         if (((blo >> 8) | bhi) == 0u && l == 0 && sh.info == 0) sh.info = tag;   // zero (or no) pivot
-        const int pl = p & 63, pq = p >> 6;
-        double rowv[16];
+        const int pl = p & 63;
+        // the owner lane hands the pivot row to the wave through LDS (in order within the wave)
 #pragma unroll
         for (int q = 0; q < RPL; q++)
-            if (q == pq) {
+            if (l + 64 * q == p) {
 #pragma unroll
-                for (int k = 0; k < 16; k++) rowv[k] = readlane_f64(v[q][k], pl);
+                for (int k = 0; k < 16; k++) sh.prow[k] = v[q][k];
             }
-        const double inv = 1.0 / rowv[kk];
+        __builtin_amdgcn_wave_barrier();
         double rs[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++) rs[k] = rowv[k] * inv;
+        for (int k = 0; k < 16; k++) rs[k] = sh.prow[k];
+        const double inv = 1.0 / rs[kk];
+#pragma unroll
+        for (int k = 0; k < 16; k++) rs[k] = rs[k] * inv;
 #pragma unroll
         for (int q = 0; q < RPL; q++) {
-            const bool isp = (l + 64 * q == p);
             const double f = v[q][kk];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const double u = fma(-f, rs[k], v[q][k]);
-                v[q][k] = isp ? rs[k] : u;
-            }
-            v[q][kk] = isp ? inv : -f * inv;
-            av[q] = av[q] && !isp;
+            for (int k = 0; k < 16; k++) v[q][k] = fma(-f, rs[k], v[q][k]);
+            v[q][kk] = -f * inv;
         }
+        // the pivot row itself: the scaled row, inv in the pivot column
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (l + 64 * q == p) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) v[q][k] = rs[k];
+                v[q][kk] = inv;
+                av[q] = false;
+            }
+        __builtin_amdgcn_wave_barrier();
         if (l == pl) sh.ipos[p] = p0 + kk;
         if (l == 0) sh.perm[p0 + kk] = p;
     }

@@ -113,8 +113,20 @@ def _pool_map(fn, items, workers=None):
     if workers <= 1 or len(items) < 16:
         return [fn(it) for it in items]
     import multiprocessing as mp
-    with mp.get_context('spawn').Pool(workers) as pool:
-        return pool.map(fn, items, chunksize=max(1, len(items) // (4 * workers)))
+    import sys
+    # a spawned worker re-imports the caller's main module unless it has no
+    # name or path; hide them so that a script which has set up the GPU is
+    # not run again in every worker (the workers only need this module)
+    main = sys.modules['__main__']
+    saved = {k: main.__dict__[k] for k in ('__spec__', '__file__') if k in main.__dict__}
+    main.__dict__.pop('__file__', None)
+    main.__spec__ = None
+    try:
+        with mp.get_context('spawn').Pool(workers) as pool:
+            return pool.map(fn, items, chunksize=max(1, len(items) // (4 * workers)))
+    finally:
+        main.__dict__.pop('__spec__', None)
+        main.__dict__.update(saved)
 
 
 def _batched_one(args):
