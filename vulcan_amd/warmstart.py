@@ -77,6 +77,22 @@ def make_case(net: Network, spec: ColumnSpec, cfg: Optional[Config] = None, ymix
     return ColumnCase(net=net, col=col, y0=col.M[:, None] * ymix0, cfg=cfg)
 
 
+def cold_case(args) -> ColumnCase:
+    """(net, spec) -> the column started from chemical equilibrium."""
+    net, spec = args
+    return make_case(net, spec)
+
+
+def warm_case(args) -> ColumnCase:
+    """(net, spec, log10 mixing ratios) -> the column started from the
+    prediction, renormalised per layer and projected onto the column's
+    element inventory."""
+    net, spec, logx = args
+    xp = 10.0 ** logx.reshape(len(spec.T), net.ni).astype(np.float64)
+    xp = xp / xp.sum(axis=1, keepdims=True)
+    return make_case(net, spec, ymix0=project(net, xp, element_vector(net, spec.logab)))
+
+
 def cold_start(net: Network, spec: ColumnSpec) -> np.ndarray:
     pco = np.logspace(np.log10(P_B), np.log10(P_T), len(spec.T))
     x = equilibrium(net, spec.T, pco, element_vector(net, spec.logab))
@@ -180,7 +196,7 @@ def run_to_steady(net: Network, cases: Sequence[ColumnCase], max_attempts: int =
         now = time.perf_counter() - t0
         newly = (st['done'] == 1) & np.isnan(wall)
         wall[newly] = now
-        if log and attempts % (poll * 64) == 0:
+        if log and attempts % 512 < poll:
             log('  attempts %d: %d/%d done, %.1f s' % (attempts, int(st['done'].sum()), nc, now))
         if st['done'].all():
             break

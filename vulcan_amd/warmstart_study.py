@@ -1,15 +1,20 @@
 """Train the warm-start MLP on solver-generated steady states and compare
 time-to-steady-state of warm and cold starts on held-out columns.
 
-    python -m vulcan_amd.warmstart_study generate --first 0 --count 2048 --out gen0.npz
-    python -m vulcan_amd.warmstart_study generate --first 2048 --count 2048 --out gen1.npz
+    python -m vulcan_amd.warmstart_study generate --first 0 --count 1024 --out gen0.npz
+    python -m vulcan_amd.warmstart_study generate --first 1024 --count 2048 --data gen0.npz --out gen1.npz
     python -m vulcan_amd.warmstart_study train-eval --data gen0.npz gen1.npz --test 512 --out report.json
 
 1. ``generate``: seeded columns (sampler in vulcan_amd/warmstart.py; column
-   i of the training stream is the same whatever the chunking) run from
-   their equilibrium cold start to VULCAN's steady state (``conv()``) in
-   batches on the device; the converged ones become training pairs
-   (features -> log10 mixing ratios, stored as float16).
+   i of the training stream is the same whatever the chunking) run to
+   VULCAN's steady state (``conv()``) in batches on the device; the
+   converged ones become training pairs (features -> log10 mixing ratios,
+   stored as float16).  The first chunk starts cold (chemical equilibrium);
+   with ``--data`` the MLP is first trained on the steady states gathered
+   so far and the new columns start from its projected prediction
+   (bootstrapped generation: the solver still decides convergence, so every
+   training target is a steady state by VULCAN's criteria, and far more of
+   them converge within the attempt cap than from equilibrium).
 2. ``train-eval``: the MLP (307 -> 4 x 1024 SiLU -> ni*nz) is trained on
    the device with the library's BF16-MFMA kernels and Adam; ``--test``
    held-out columns (a different seed stream) then run twice to steady
@@ -36,19 +41,6 @@ def log(msg):
     print('[warmstart] ' + msg, flush=True)
 
 
-def _cold_case(args):
-    net, spec = args
-    return WS.make_case(net, spec)
-
-
-def _warm_case(args):
-    net, spec, logx = args
-    xp = 10.0 ** logx.reshape(len(spec.T), net.ni).astype(np.float64)
-    xp = xp / xp.sum(axis=1, keepdims=True)
-    xw = WS.project(net, xp, WS.element_vector(net, spec.logab))
-    return WS.make_case(net, spec, ymix0=xw)
-
-
 def _cases(fn, net, items):
     out = _pool_map(fn, [(net,) + tuple(it) for it in items])
     for c in out:
@@ -56,22 +48,48 @@ def _cases(fn, net, items):
     return out
 
 
-def generate(first, count, seed, nz, batch, max_attempts, poll, out):
+def load_data(files):
+    parts = [np.load(f) for f in files]
+    X = np.concatenate([p['X'][p['done']] for p in parts]).astype(np.float64)
+    Y = np.concatenate([p['Y'][p['done']] for p in parts]).astype(np.float64)
+    info = {'generated': int(sum(len(p['done']) for p in parts)),
+            'attempts_converged': np.concatenate([p['attempts'][p['done']] for p in parts]),
+            'nz': int(parts[0]['nz'])}
+    return X, Y, info
+
+
+def generate(first, count, seed, nz, batch, max_attempts, poll, out, data=None, epochs=200, hidden=1024,
+             layers=4, min_train=64, time_budget=1e9):
     net = full_network()
     from .device import DevicePlan
     plan = DevicePlan(net)
     specs = WS.sample_specs(first + count, seed, nz)[first:]
+    mlp = norm = None
+    if data:
+        Xd, Yd, info = load_data(data)
+        if Xd.shape[0] >= min_train:
+            log('training the warm-start MLP on %d steady states' % Xd.shape[0])
+            mlp, norm, _ = train([Xd.shape[1]] + [hidden] * layers + [Yd.shape[1]], Xd, Yd, epochs, 256, 1e-3)
     X = WS.features(specs).astype(np.float32)
     Y = np.zeros((count, nz * net.ni), np.float16)
     rec = {k: np.zeros(count, t) for k, t in (('done', bool), ('count', np.int32), ('attempts', np.int32),
-                                             ('wall', np.float32), ('t_model', np.float64))}
+                                             ('wall', np.float32), ('t_model', np.float64), ('warm', bool))}
     t_all = time.perf_counter()
+    n_done = 0
     for b0 in range(0, count, batch):
+        if time.perf_counter() - t_all > time_budget:
+            log('time budget reached after %d columns' % b0)
+            break
         chunk = specs[b0:b0 + batch]
         t0 = time.perf_counter()
-        cases = _cases(_cold_case, net, [(s,) for s in chunk])
-        log('batch %d-%d: %d cold starts built in %.1f s' % (first + b0, first + b0 + len(chunk), len(chunk),
-                                                             time.perf_counter() - t0))
+        if mlp is not None:
+            P = predict(mlp, norm, WS.features(chunk))
+            cases = _cases(WS.warm_case, net, list(zip(chunk, P)))
+        else:
+            cases = _cases(WS.cold_case, net, [(s,) for s in chunk])
+        log('batch %d-%d: %d %s starts built in %.1f s' % (first + b0, first + b0 + len(chunk), len(chunk),
+                                                           'warm' if mlp is not None else 'cold',
+                                                           time.perf_counter() - t0))
         res = WS.run_to_steady(net, cases, max_attempts=max_attempts, poll=poll, plan=plan, log=log)
         sl = slice(b0, b0 + len(chunk))
         rec['done'][sl] = res.done
@@ -79,13 +97,16 @@ def generate(first, count, seed, nz, batch, max_attempts, poll, out):
         rec['attempts'][sl] = res.attempts
         rec['wall'][sl] = res.wall
         rec['t_model'][sl] = res.t_model
+        rec['warm'][sl] = mlp is not None
         Y[sl] = np.stack([WS.targets(ym[None])[0] for ym in res.ymix]).astype(np.float16)
-        log('batch %d-%d: %d/%d converged, median steps %d, %.1f s' % (
-            first + b0, first + b0 + len(chunk), int(res.done.sum()), len(chunk), int(np.median(res.count)),
-            time.perf_counter() - t0))
-    np.savez_compressed(out, index=np.arange(first, first + count, dtype=np.int32), X=X, Y=Y, seed=seed, nz=nz,
-                        **rec)
-    log('generated %d columns (%d converged) in %.1f s -> %s' % (count, int(rec['done'].sum()),
+        n_done = b0 + len(chunk)
+        log('batch %d-%d: %d/%d converged, median attempts of converged %s, %.1f s' % (
+            first + b0, first + n_done, int(res.done.sum()), len(chunk),
+            int(np.median(res.attempts[res.done])) if res.done.any() else None, time.perf_counter() - t0))
+    sl = slice(0, n_done)
+    np.savez_compressed(out, index=np.arange(first, first + n_done, dtype=np.int32), X=X[sl], Y=Y[sl], seed=seed,
+                        nz=nz, **{k: v[sl] for k, v in rec.items()})
+    log('generated %d columns (%d converged) in %.1f s -> %s' % (n_done, int(rec['done'][sl].sum()),
                                                                 time.perf_counter() - t_all, out))
 
 
@@ -128,8 +149,8 @@ def predict(mlp, norm, X):
 def compare(net, specs, P, max_attempts, poll, plan=None):
     """Cold (equilibrium) and warm (projected prediction P, log10 mixing
     ratios) runs of the same columns to steady state, one batch each."""
-    cold = _cases(_cold_case, net, [(s,) for s in specs])
-    warm = _cases(_warm_case, net, [(s, p) for s, p in zip(specs, P)])
+    cold = _cases(WS.cold_case, net, [(s,) for s in specs])
+    warm = _cases(WS.warm_case, net, [(s, p) for s, p in zip(specs, P)])
     log('cold runs (%d columns)' % len(specs))
     rc = WS.run_to_steady(net, cold, max_attempts=max_attempts, poll=poll, plan=plan, log=log)
     log('warm runs (%d columns)' % len(specs))
@@ -159,12 +180,10 @@ def compare(net, specs, P, max_attempts, poll, plan=None):
 
 def train_eval(data, test, test_seed, hidden, layers, epochs, mb, lr, max_attempts, poll, out, save_model=None):
     net = full_network()
-    parts = [np.load(f) for f in data]
-    nz = int(parts[0]['nz'])
-    X = np.concatenate([p['X'][p['done']] for p in parts]).astype(np.float64)
-    Y = np.concatenate([p['Y'][p['done']] for p in parts]).astype(np.float64)
-    n_gen = int(sum(len(p['done']) for p in parts))
-    gen_attempts = np.concatenate([p['attempts'][p['done']] for p in parts])
+    X, Y, info = load_data(data)
+    nz = info['nz']
+    n_gen = info['generated']
+    gen_attempts = info['attempts_converged']
     log('training on %d converged columns of %d generated' % (X.shape[0], n_gen))
     sizes = [X.shape[1]] + [hidden] * layers + [Y.shape[1]]
     torch.cuda.synchronize()
@@ -207,6 +226,10 @@ def main(argv=None):
     g.add_argument('--max-attempts', type=int, default=8000)
     g.add_argument('--poll', type=int, default=16)
     g.add_argument('--out', required=True)
+    g.add_argument('--data', nargs='*', default=None, help='earlier outputs: train on them and start warm')
+    g.add_argument('--epochs', type=int, default=200)
+    g.add_argument('--min-train', type=int, default=64)
+    g.add_argument('--time-budget', type=float, default=1e9, help='seconds; no new batch starts after it')
     t = sub.add_parser('train-eval')
     t.add_argument('--data', nargs='+', required=True)
     t.add_argument('--test', type=int, default=512)
@@ -222,7 +245,8 @@ def main(argv=None):
     t.add_argument('--save-model', default=None)
     a = ap.parse_args(argv)
     if a.cmd == 'generate':
-        generate(a.first, a.count, a.seed, a.nz, a.batch, a.max_attempts, a.poll, a.out)
+        generate(a.first, a.count, a.seed, a.nz, a.batch, a.max_attempts, a.poll, a.out, data=a.data,
+                 epochs=a.epochs, min_train=a.min_train, time_budget=a.time_budget)
     else:
         train_eval(a.data, a.test, a.test_seed, a.hidden, a.layers, a.epochs, a.mb, a.lr, a.max_attempts, a.poll,
                    a.out, a.save_model)

@@ -108,7 +108,8 @@ def iso_gradient_profile(nz: int, T_top: float, T_bot: float, P_b=1e9, P_t=1e4, 
 def _pool_map(fn, items, workers=None):
     """Map over case descriptions, in worker processes (spawned: no state is
     inherited from the caller, which may hold a GPU context) when there are
-    many; results come back in order."""
+    many; results come back in order.  `fn` must live in an importable
+    module (the workers do not import the caller's main script)."""
     workers = workers or min(16, os.cpu_count() or 1, len(items))
     if workers <= 1 or len(items) < 16:
         return [fn(it) for it in items]
@@ -121,9 +122,11 @@ def _pool_map(fn, items, workers=None):
     saved = {k: main.__dict__[k] for k in ('__spec__', '__file__') if k in main.__dict__}
     main.__dict__.pop('__file__', None)
     main.__spec__ = None
+    from concurrent.futures import ProcessPoolExecutor
     try:
-        with mp.get_context('spawn').Pool(workers) as pool:
-            return pool.map(fn, items, chunksize=max(1, len(items) // (4 * workers)))
+        # a worker failure raises here instead of leaving the map waiting
+        with ProcessPoolExecutor(workers, mp_context=mp.get_context('spawn')) as ex:
+            return list(ex.map(fn, items, chunksize=max(1, len(items) // (4 * workers))))
     finally:
         main.__dict__.pop('__spec__', None)
         main.__dict__.update(saved)
