@@ -317,11 +317,11 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         const unsigned bhi = (unsigned)(best >> 32);
         const int p = (blo | bhi) ? 255 - (int)(blo & 0xffu) : 0;
         if (((blo >> 8) | bhi) == 0u && l == 0 && sh.info == 0) sh.info = tag;   // zero (or no) pivot
-        const int pl = p & 63;
+        const int pl = p & 63, pq = p >> 6;   // owner lane and register row slot (uniform)
         // the owner lane hands the pivot row to the wave through LDS (in order within the wave)
 #pragma unroll
         for (int q = 0; q < RPL; q++)
-            if (l + 64 * q == p) {
+            if (q == pq && l == pl) {
 #pragma unroll
                 for (int k = 0; k < 16; k++) sh.prow[k] = v[q][k];
             }
@@ -342,7 +342,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         // the pivot row itself: the scaled row, inv in the pivot column
 #pragma unroll
         for (int q = 0; q < RPL; q++)
-            if (l + 64 * q == p) {
+            if (q == pq && l == pl) {
 #pragma unroll
                 for (int k = 0; k < 16; k++) v[q][k] = rs[k];
                 v[q][kk] = inv;
