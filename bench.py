@@ -33,6 +33,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # B/s
+VARIANTS = ['register Gauss-Jordan chain', 'blocked Gauss-Jordan chain (MFMA Schur updates)',
+            'blocked Gauss-Jordan chain, two workgroups per CU', 'two-wave Gauss-Jordan chain',
+            'block cyclic reduction (MFMA Schur updates)']
 FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
 
 
@@ -82,8 +85,10 @@ def bench_single(steps, warmup, graph=True):
     st = b.host_state()[0]
     factor_ms = factor_timing(b, 20)
     nz, ni = c.col.nz, c.net.ni
+    variant = b._lib.vk_plan_factor_variant(plan.handle)
     return dict(time=dt, steps=steps, attempts=attempts, nz=nz, ni=ni, npairs=c.net.npairs,
-                factor_ms=factor_ms, t_model=float(st['t']), dt_model=float(st['dt']))
+                factor_ms=factor_ms, t_model=float(st['t']), dt_model=float(st['dt']), variant=variant,
+                tuner_ms=[round(v, 3) for v in b.factor_ms])
 
 
 def bench_batched(attempts, warmup, ncols=256, graph=True):
@@ -210,10 +215,12 @@ def main():
         'config': {'workload': 'plain method, one column: nz=150, ni=93, 600 pairs (90 Troe) + 20 photolysis, '
                                'dt0=1e-10 s', 'attempts': s['attempts'], 'model_time_s': s['t_model'],
                    'launch': 'eager' if args.no_graph else 'hip-graph per attempt'},
-        'roofline': {'kernel': 'k_factor (block-Thomas Gauss-Jordan inversion)', 'bound': 'mfma',
+        'roofline': {'kernel': 'block factorisation of the Ros2 lhs: ' + VARIANTS[s['variant']]
+                               + ' (chosen by the tuner; per-variant ms for factor + 2 solves: %s)' % s['tuner_ms'],
+                     'bound': 'mfma',
                      'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': achieved / FP64_PEAK, 'traffic': None,
-                     'flops_per_launch': flops_launch, 'avg_launch_ms': s['factor_ms']},
+                     'flops_per_factorisation': flops_launch, 'avg_factorisation_ms': s['factor_ms']},
     }
     if not args.no_batched:
         print('[bench] batched 256 columns: %d attempts' % args.batched_attempts, file=sys.stderr, flush=True)

@@ -169,19 +169,34 @@ int vk_block_factor(const vk_plan *plan, const vk_columns *cols, const double *d
  * FP64 MFMA (compiled for one or for two resident workgroups per CU);
  * VK_FACTOR_WAVE holds the block in the registers of two waves and runs
  * one two-wave barrier per pivot step (small workgroups: several chains per
- * CU).  All give the inverse to rounding; vk_tune_factor times every
+ * CU).  VK_FACTOR_BCR replaces the chain over layers by log2(nz) levels of
+ * block cyclic reduction (independent inversions per level, dense
+ * Schur-complement updates on FP64 MFMA): far shorter dependence chains for
+ * few columns; it keeps its own workspace, so it runs inside the
+ * integrator and vk_block_solve_system, while vk_block_factor /
+ * vk_block_solve reject it.  All give the solution to rounding; vk_tune_factor times every
  * variant on a synthetic system of the columns' shape (`reps` launches
- * each, milliseconds per launch into ms[VK_FACTOR_NVARIANTS], -1 for a
- * variant the plan's ni does not allow) and keeps the
+ * each: one factorisation and two solves, milliseconds per repetition into
+ * ms[VK_FACTOR_NVARIANTS], -1 for a variant that does not apply: the
+ * two-wave kernel above ni = 112, cyclic reduction above 64 columns) and
+ * keeps the
  * fastest in the plan. */
 #define VK_FACTOR_REGISTER     0
 #define VK_FACTOR_BLOCKED      1   /* register budget for one workgroup per CU      */
 #define VK_FACTOR_BLOCKED_OCC2 2   /* register budget for two workgroups per CU     */
 #define VK_FACTOR_WAVE         3   /* two waves per block, unblocked (ni <= 112)    */
-#define VK_FACTOR_NVARIANTS    4
+#define VK_FACTOR_BCR          4   /* block cyclic reduction across layers          */
+#define VK_FACTOR_NVARIANTS    5
 int vk_plan_set_factor_variant(vk_plan *plan, int variant);
 int vk_plan_factor_variant(const vk_plan *plan);
 int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, float *ms, void *stream);
+/* x = lhs^-1 b for the block system (scipy.linalg.solve_banded at op.py:2914,
+ * 2929, with `refine` steps of double-double iterative refinement), any
+ * factor variant; allocates its workspace on the stream.  info as for
+ * vk_block_factor. */
+int vk_block_solve_system(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                          const double *d_Lsub, const double *d_Usup, const double *d_b, double *d_x,
+                          int refine, int *d_info, void *stream);
 /* Self-test of the FP64 MFMA operand layout the blocked inversion relies
  * on: out[i*16+j] = sum_k A[i][k] B[k][j] with A[i][k] = i + k/4 + 1,
  * B[k][j] = 1 - k/2 + j/8 (16 x 4 x 16), computed by one MFMA. */

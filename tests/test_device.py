@@ -182,7 +182,7 @@ def test_wave_max_u64():
         assert (out.cpu().numpy() == v.max()).all(), trial
 
 
-FACTOR_VARIANTS = [0, 1, 2, 3]   # register Gauss-Jordan, blocked MFMA (1 or 2 workgroups per CU), two-wave
+FACTOR_VARIANTS = [0, 1, 2, 3, 4]   # register GJ, blocked MFMA (1 or 2 WGs per CU), two-wave, cyclic reduction
 
 
 @gpu
@@ -209,8 +209,27 @@ def test_block_factor_solve_against_dense(ni, nz, variant):
     cols = DeviceColumns([nz])
     T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device='cuda')
     Dd, Ld, Ud, bd = T(D), T(L), T(U), T(b)
-    Di = torch.empty_like(Dd)
+    Al = A.astype(np.longdouble)
+    res = lambda v: (b.ravel().astype(np.longdouble) - Al @ v.ravel().astype(np.longdouble))
     info = torch.zeros(1, dtype=torch.int32, device='cuda')
+    # the whole solve, with and without one refinement step (every variant)
+    xs = []
+    for refine in (0, 1):
+        xo = torch.empty_like(bd)
+        _lib.check(lib.vk_block_solve_system(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(bd),
+                                             dptr(xo), refine, dptr(info), stream_handle()), 'vk_block_solve_system')
+        xs.append(xo.cpu().numpy())
+    assert int(info.cpu()[0]) == 0
+    assert np.array_equal(Dd.cpu().numpy(), D)          # the blocks are not modified
+    assert np.max(np.abs(xs[0] - x)) <= 1e-12 * np.abs(x).max()
+    assert np.max(np.abs(res(xs[1]))) <= np.max(np.abs(res(xs[0]))) + 1e-15 * np.abs(b).max()
+    if variant == 4:
+        # the factor/solve pair of the public ABI needs a workspace the cyclic reduction keeps itself
+        Di = torch.empty_like(Dd)
+        assert lib.vk_block_factor(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(Di), None,
+                                   stream_handle()) == -1001
+        return
+    Di = torch.empty_like(Dd)
     _lib.check(lib.vk_block_factor(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(Di), dptr(info),
                                    stream_handle()), 'vk_block_factor')
     xd_ = bd.clone()
@@ -218,15 +237,12 @@ def test_block_factor_solve_against_dense(ni, nz, variant):
                                   stream_handle()), 'vk_block_solve')
     xd = xd_.cpu().numpy()
     assert int(info.cpu()[0]) == 0
-    assert np.array_equal(Dd.cpu().numpy(), D)          # the blocks are not modified
     assert np.max(np.abs(xd - x)) <= 1e-12 * np.abs(x).max()
     # one refinement step with the double-double residual
     r = torch.empty_like(bd)
     _lib.check(lib.vk_block_residual(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(xd_), dptr(bd),
                                      dptr(r), stream_handle()), 'vk_block_residual')
     rd = r.cpu().numpy()
-    Al = A.astype(np.longdouble)
-    res = lambda v: (b.ravel().astype(np.longdouble) - Al @ v.ravel().astype(np.longdouble))
     r_ref = res(xd).astype(np.float64)
     assert np.max(np.abs(rd.ravel() - r_ref)) <= 1e-3 * np.abs(r_ref).max() + 1e-300
     _lib.check(lib.vk_block_solve(plan.handle, cols.handle, dptr(Di), dptr(Ld), dptr(Ud), dptr(r),
@@ -235,6 +251,59 @@ def test_block_factor_solve_against_dense(ni, nz, variant):
     # the refined solution has the smaller residual (numpy's own solution is
     # no exact reference at this conditioning)
     assert np.max(np.abs(res(x2))) <= np.max(np.abs(res(xd))) + 1e-15 * np.abs(b).max()
+
+
+@gpu
+def test_cyclic_reduction_ragged_batch_matches_dense():
+    """Cyclic reduction over columns of different heights (1 to 9 levels)."""
+    torch = _torch()
+    from vulcan_amd.device import DeviceColumns, DevicePlan, dptr, stream_handle
+    ni = 21
+    nzs = [2, 3, 5, 8, 13, 150, 4]
+    rng = np.random.default_rng(5)
+    plan = DevicePlan(_sized_network(ni))
+    lib = _lib.load()
+    _lib.check(lib.vk_plan_set_factor_variant(plan.handle, 4), 'vk_plan_set_factor_variant')
+    cols = DeviceColumns(nzs)
+    Ds, Ls, Us, bs, xs = [], [], [], [], []
+    for nz in nzs:
+        D = rng.standard_normal((nz, ni, ni)) + 6 * np.eye(ni)
+        L = rng.standard_normal((nz, ni)); L[0] = 0
+        U = rng.standard_normal((nz, ni)); U[-1] = 0
+        b = rng.standard_normal((nz, ni))
+        xs.append(np.linalg.solve(O.lhs_dense(D, L, U), b.ravel()).reshape(nz, ni))
+        Ds.append(D); Ls.append(L); Us.append(U); bs.append(b)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(np.concatenate(a)), dtype=torch.float64, device='cuda')
+    Dd, Ld, Ud, bd = T(Ds), T(Ls), T(Us), T(bs)
+    xo = torch.empty_like(bd)
+    info = torch.zeros(len(nzs), dtype=torch.int32, device='cuda')
+    _lib.check(lib.vk_block_solve_system(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(bd), dptr(xo),
+                                         0, dptr(info), stream_handle()), 'vk_block_solve_system')
+    assert not info.cpu().numpy().any()
+    x = np.concatenate(xs)
+    assert np.max(np.abs(xo.cpu().numpy() - x)) <= 1e-12 * np.abs(x).max()
+
+
+@gpu
+def test_integration_with_cyclic_reduction_matches_chain():
+    """The same batch integrated with cyclic reduction and with the chained
+    block Thomas factorisation agrees to rounding over 20 attempts."""
+    torch = _torch()
+    cases = W.batched_cases(3, nz_range=(12, 30))
+    net = cases[0].net
+    runs = []
+    for variant in (0, 4):
+        b = _batch(net, [c.col for c in cases], [c.y0 for c in cases], cases[0].cfg)
+        _lib.check(_lib.load().vk_plan_set_factor_variant(b.plan.handle, variant), 'vk_plan_set_factor_variant')
+        b.set_graph(True)
+        b.integrate(20, use_conv=False)
+        torch.cuda.synchronize()
+        runs.append((b.y.cpu().numpy(), b.host_state()))
+    (y0, s0), (y4, s4) = runs
+    assert np.array_equal(s0['count'], s4['count'])
+    rel, absn = step_err(y4, y0, y0.sum(axis=1))
+    print('cyclic reduction vs chain after 20 attempts: rel %.3e abs %.3e' % (rel, absn))
+    assert rel < 1e-9
 
 
 @gpu
@@ -255,8 +324,14 @@ def test_zero_pivot_is_reported():
         Dd, Ld, Ud = T(D), T(L), T(U)
         Di = torch.empty_like(Dd)
         info = torch.zeros(1, dtype=torch.int32, device='cuda')
-        _lib.check(lib.vk_block_factor(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(Di), dptr(info),
-                                       stream_handle()), 'vk_block_factor')
+        if variant == 4:
+            b = torch.ones((nz, ni), dtype=torch.float64, device='cuda')
+            xo = torch.empty_like(b)
+            _lib.check(lib.vk_block_solve_system(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(b),
+                                                 dptr(xo), 0, dptr(info), stream_handle()), 'vk_block_solve_system')
+        else:
+            _lib.check(lib.vk_block_factor(plan.handle, cols.handle, dptr(Dd), dptr(Ld), dptr(Ud), dptr(Di),
+                                           dptr(info), stream_handle()), 'vk_block_factor')
         assert int(info.cpu()[0]) == 2, variant
 
 
@@ -272,13 +347,13 @@ def test_factor_tuner_picks_the_fastest_variant():
     _lib.check(lib.vk_tune_factor(plan.handle, cols.handle, 2, ms, stream_handle()), 'vk_tune_factor')
     t = list(ms)
     print('factor variants ms:', t)
-    assert all(v > 0 for v in t)                       # ni = 93: every variant applies
+    assert all(v > 0 for v in t)                       # ni = 93, 3 columns: every variant applies
     assert lib.vk_plan_factor_variant(plan.handle) == int(np.argmin(t))
     big = DevicePlan(_sized_network(150))
     big_cols = DeviceColumns([7])
     ms2 = (C.c_float * len(FACTOR_VARIANTS))()
     _lib.check(lib.vk_tune_factor(big.handle, big_cols.handle, 1, ms2, stream_handle()), 'vk_tune_factor')
-    assert ms2[3] == -1.0 and all(ms2[v] > 0 for v in (0, 1, 2))
+    assert ms2[3] == -1.0 and all(ms2[v] > 0 for v in (0, 1, 2, 4))
     assert lib.vk_plan_set_factor_variant(big.handle, 3) == -1001        # VK_EINVAL
 
 

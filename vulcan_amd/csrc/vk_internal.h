@@ -36,11 +36,31 @@ struct vk_plan {
     std::vector<void *> allocs;
 };
 
+// Block cyclic reduction schedule of a set of columns: at level s the
+// active layers of each column at odd positions are eliminated, those at
+// even positions survive; one layer per column remains on top.
+struct vk_bcr_sched {
+    int nlev = 0;
+    std::vector<int> elim_off, surv_off;   // nlev+1 each, into the entry arrays
+    int nsurv = 0, ntop = 0;
+    int4 *d_elim = nullptr;   // (layer, left, right or -1, column)
+    int4 *d_surv = nullptr;   // (layer, left or -1, right or -1, column)
+    int *d_sflag = nullptr;   // per surviving entry: bit 0 new left coupling, bit 1 new right coupling
+    int2 *d_top = nullptr;    // (layer, column)
+};
+
+// workspace of the block cyclic reduction: A, B, C per layer (n x n each),
+// P, Q per surviving (level, layer) entry
+struct vk_bcr_ws {
+    double *A = nullptr, *B = nullptr, *C = nullptr, *P = nullptr, *Q = nullptr;
+};
+
 struct vk_columns {
     int ncols, L, max_nz;
     std::vector<int> off;      // host copy, ncols+1
     int *d_off;                // ncols+1
     int *d_layer_col;          // L
+    vk_bcr_sched bcr;
 };
 
 // per-layer partial results of the finalize kernel
@@ -56,6 +76,7 @@ struct vk_integrator {
     double *ztmp;                       // forward-sweep scratch of the twisted solve
     double *yk2, *part;
     double *ysum;                       // layer totals for the diffusion terms
+    vk_bcr_ws bcr;                      // block cyclic reduction workspace (allocated when selected)
     int *info;
     double *yhist;             // [history][L][ni]
     double *thist;             // [ncols][tcap]

@@ -4,6 +4,7 @@
 // (op.py:2447-2534), update_mu_dz (944-984), renormalisation (909-914),
 // save_step (1087-1105), step_size (3105-3125) and stop/conv (1018-1085).
 #include <cstring>
+#include <type_traits>
 #include "vk_internal.h"
 #include "vk_device.h"
 
@@ -15,9 +16,9 @@ int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const
                       const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
                       double *, hipStream_t);
 int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
-                     int *, const vk_colstate *, hipStream_t);
+                     int *, const vk_colstate *, const vk_bcr_ws *, hipStream_t);
 int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
-                            const double *, const double *, double *, double *, double *, int, const vk_colstate *,
+                            const double *, const double *, double *, double *, double *, int, const vk_colstate *, const vk_bcr_ws *,
                             hipStream_t);
 
 namespace {
@@ -476,6 +477,60 @@ extern "C" int vk_plan_info(const vk_plan *p, int *ni, int *nr, int *max_ni) {
     return 0;
 }
 
+// levels of block cyclic reduction for every column (see vk_bcr_sched)
+static int bcr_schedule(vk_columns *c) {
+    vk_bcr_sched &b = c->bcr;
+    std::vector<std::vector<int>> act(c->ncols);
+    for (int i = 0; i < c->ncols; i++)
+        for (int l = c->off[i]; l < c->off[i + 1]; l++) act[i].push_back(l);
+    std::vector<int4> el, sv;
+    std::vector<int> fl;
+    std::vector<int2> top;
+    b.elim_off.push_back(0);
+    b.surv_off.push_back(0);
+    for (;;) {
+        bool any = false;
+        for (int i = 0; i < c->ncols; i++) {
+            const std::vector<int> &a = act[i];
+            const int n = (int)a.size();
+            if (n < 2) continue;
+            any = true;
+            for (int q = 1; q < n; q += 2) el.push_back(make_int4(a[q], a[q - 1], q + 1 < n ? a[q + 1] : -1, i));
+            for (int q = 0; q < n; q += 2) {
+                const int left = q - 1 >= 0 ? a[q - 1] : -1, right = q + 1 < n ? a[q + 1] : -1;
+                sv.push_back(make_int4(a[q], left, right, i));
+                fl.push_back((left >= 0 ? 1 : 0) | (q + 2 < n ? 2 : 0));
+            }
+        }
+        if (!any) break;
+        for (int i = 0; i < c->ncols; i++) {
+            if (act[i].size() < 2) continue;
+            std::vector<int> keep;
+            for (size_t q = 0; q < act[i].size(); q += 2) keep.push_back(act[i][q]);
+            act[i] = keep;
+        }
+        b.elim_off.push_back((int)el.size());
+        b.surv_off.push_back((int)sv.size());
+        b.nlev++;
+    }
+    for (int i = 0; i < c->ncols; i++) top.push_back(make_int2(act[i][0], i));
+    b.nsurv = (int)sv.size();
+    b.ntop = (int)top.size();
+    auto put = [](auto **dst, const auto &vec) -> int {
+        using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+        const size_t n = vec.empty() ? 1 : vec.size();
+        if (hipMalloc((void **)dst, sizeof(T) * n) != hipSuccess) return VK_ENOMEM;
+        if (!vec.empty() && hipMemcpy(*dst, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return VK_ENOMEM;
+        return 0;
+    };
+    int r = put(&b.d_elim, el);
+    r = r ? r : put(&b.d_surv, sv);
+    r = r ? r : put(&b.d_sflag, fl);
+    r = r ? r : put(&b.d_top, top);
+    return r;
+}
+
 extern "C" int vk_columns_create(int ncols, const int *nz, vk_columns **out) {
     if (ncols <= 0 || !nz || !out) return VK_EINVAL;
     vk_columns *c = new vk_columns();
@@ -498,6 +553,8 @@ extern "C" int vk_columns_create(int ncols, const int *nz, vk_columns **out) {
     }
     VK_CHECK(hipMemcpy(c->d_off, c->off.data(), sizeof(int) * (ncols + 1), hipMemcpyHostToDevice));
     VK_CHECK(hipMemcpy(c->d_layer_col, lc.data(), sizeof(int) * c->L, hipMemcpyHostToDevice));
+    int r = bcr_schedule(c);
+    if (r) { vk_columns_destroy(c); return r; }
     *out = c;
     return 0;
 }
@@ -506,6 +563,10 @@ extern "C" int vk_columns_destroy(vk_columns *c) {
     if (!c) return 0;
     (void)hipFree(c->d_off);
     (void)hipFree(c->d_layer_col);
+    if (c->bcr.d_elim) (void)hipFree(c->bcr.d_elim);
+    if (c->bcr.d_surv) (void)hipFree(c->bcr.d_surv);
+    if (c->bcr.d_sflag) (void)hipFree(c->bcr.d_sflag);
+    if (c->bcr.d_top) (void)hipFree(c->bcr.d_top);
     delete c;
     return 0;
 }
@@ -563,6 +624,20 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
     it->factor_launches = 0;
     *out = it;
     return 0;
+}
+
+// the cyclic reduction workspace, allocated the first time the plan selects it
+static int ensure_bcr(vk_integrator *it) {
+    if (it->plan->factor_variant != VK_FACTOR_BCR || it->bcr.A) return 0;
+    const size_t nn = (size_t)it->plan->ni * it->plan->ni, L = (size_t)it->cols->L;
+    const size_t ns = (size_t)it->cols->bcr.nsurv + 1;
+    auto &v = it->allocs;
+    int r = dalloc(v, &it->bcr.A, L * nn);
+    r = r ? r : dalloc(v, &it->bcr.B, L * nn);
+    r = r ? r : dalloc(v, &it->bcr.C, L * nn);
+    r = r ? r : dalloc(v, &it->bcr.P, ns * nn);
+    r = r ? r : dalloc(v, &it->bcr.Q, ns * nn);
+    return r;
 }
 
 static int ensure_thist(vk_integrator *it, int count_max) {
@@ -624,11 +699,11 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
                           s);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
-    r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, s);
+    r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, &it->bcr, s);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev1, s));
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b1, it->x1, it->work, it->ztmp, prm->refine,
-                                st, s);   // k1
+                                st, &it->bcr, s);   // k1
     if (r) return r;
     const int n = (int)(L * ni);
     hipLaunchKernelGGL(k_yk2, dim3((n + 255) / 256), dim3(256), 0, s, n, ni, cl->d_layer_col, y, it->x1, st, it->yk2);
@@ -636,7 +711,7 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
                       s);   // f(yk2) - 2/(r h) k1
     if (r) return r;
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b2, it->x2, it->work, it->ztmp, prm->refine,
-                                st, s);   // k2
+                                st, &it->bcr, s);   // k2
     if (r) return r;
     hipLaunchKernelGGL(k_finalize, dim3(cl->L), dim3(256), 0, s, ni, p->natom, cl->d_off, cl->d_layer_col,
                        p->atom_mat, y, ymix, it->x1, it->x2, it->yk2, *prm, clip, st, y, ymix, it->part);
@@ -656,7 +731,9 @@ extern "C" int vk_ros2_solver(vk_integrator *it, const vk_atm *atm, const vk_par
                               double *d_y, double *d_ymix, vk_colstate *d_st, void *stream) {
     if (!it || !atm || !prm || !d_k || !d_y || !d_ymix || !d_st) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
-    int r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 0, false, s);
+    int r = ensure_bcr(it);
+    if (r) return r;
+    r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 0, false, s);
     if (r) return r;
     hipLaunchKernelGGL(k_decide, dim3(it->cols->ncols), dim3(1), 0, s, it->plan->natom, it->cols->d_off, it->part,
                        *prm, 0, nullptr, nullptr, nullptr, d_st, nullptr, nullptr, 0);
@@ -705,6 +782,7 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     if (prm->use_conv && it->history > 0 && it->history < prm->conv_step + 1) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     int r = ensure_thist(it, prm->count_max);
+    if (!r) r = ensure_bcr(it);
     if (r) return r;
     vk_geom g{};
     if (geom) g = *geom;

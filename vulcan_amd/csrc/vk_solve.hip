@@ -317,11 +317,11 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         const unsigned bhi = (unsigned)(best >> 32);
         const int p = (blo | bhi) ? 255 - (int)(blo & 0xffu) : 0;
         if (((blo >> 8) | bhi) == 0u && l == 0 && sh.info == 0) sh.info = tag;   // zero (or no) pivot
-        const int pl = p & 63, pq = p >> 6;   // owner lane and register row slot (uniform)
+        const int pl = p & 63;
         // the owner lane hands the pivot row to the wave through LDS (in order within the wave)
 #pragma unroll
         for (int q = 0; q < RPL; q++)
-            if (q == pq && l == pl) {
+            if (l + 64 * q == p) {
 #pragma unroll
                 for (int k = 0; k < 16; k++) sh.prow[k] = v[q][k];
             }
@@ -342,7 +342,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         // the pivot row itself: the scaled row, inv in the pivot column
 #pragma unroll
         for (int q = 0; q < RPL; q++)
-            if (q == pq && l == pl) {
+            if (l + 64 * q == p) {
 #pragma unroll
                 for (int k = 0; k < 16; k++) v[q][k] = rs[k];
                 v[q][kk] = inv;
@@ -770,6 +770,285 @@ __global__ void __launch_bounds__(NTW) k_factor_wave_mid(int ni, const int *__re
     if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
 }
 
+// ---------------------------------------------------------------------------
+// Block cyclic reduction (factor variant 4)
+//
+// Every level eliminates the odd-positioned active layers of each column at
+// once: their blocks are inverted in parallel (blk_invert), and each
+// surviving layer j with eliminated neighbours l (left) and r (right) takes
+//     P_j = B_j A_l^-1,   Q_j = C_j A_r^-1,
+//     A_j -= P_j C_l + Q_j B_r,   B_j = -P_j B_l,   C_j = -Q_j C_r
+// (B: coupling to the previous active layer, C: to the next; diagonal at
+// level 0, dense above).  The chain over layers becomes log2(nz) levels of
+// independent work: the Schur-complement updates above level 0 are dense
+// n x n products on FP64 MFMA, level 0 is elementwise.  P and Q are kept
+// for the right-hand side reduction of the solves.
+// ---------------------------------------------------------------------------
+
+// acc += op(A) op(B) over k < ni, A(a, c) = Ab[a*sar + c*sac], B(c, b) = Bb[c*sbr + b*sbc],
+// operands staged through LDS in slices of 16 (tile layout of blk_invert).
+template <int NB>
+struct GemmShared {
+    double sa[16][16 * NB + 1];
+    double sb[16][16 * NB];
+};
+
+template <int NB>
+__device__ __forceinline__ void blk_gemm_acc(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni,
+                                             const double *__restrict__ Ab, long sar, long sac,
+                                             const double *__restrict__ Bb, long sbr, long sbc, GemmShared<NB> &g) {
+    constexpr int N16 = 16 * NB, TB = (NB + 1) / 2;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int wr = w >> 1, wc = w & 1, lc = l & 15;
+    for (int k0 = 0; k0 < ni; k0 += 16) {
+        __syncthreads();
+        for (int e = tid; e < 16 * N16; e += NT) {
+            const int kk = e / N16, rr = e % N16, c = k0 + kk;
+            g.sa[kk][rr] = (rr < ni && c < ni) ? Ab[(long)rr * sar + (long)c * sac] : 0.0;
+            g.sb[kk][rr] = (rr < ni && c < ni) ? Bb[(long)c * sbr + (long)rr * sbc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ti = 0; ti < TB; ti++) {
+            const int bi = wr + 2 * ti;
+            if (bi >= NB) continue;
+            double a[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) a[s4] = g.sa[4 * s4 + (l >> 4)][16 * bi + lc];
+#pragma unroll
+            for (int tj = 0; tj < TB; tj++) {
+                const int bj = wc + 2 * tj;
+                if (bj >= NB) continue;
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s4], g.sb[4 * s4 + (l >> 4)][16 * bj + lc],
+                                                                        acc[ti][tj], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int NB>
+__device__ __forceinline__ void blk_zero(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2]) {
+#pragma unroll
+    for (int ti = 0; ti < (NB + 1) / 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < (NB + 1) / 2; tj++) acc[ti][tj] = f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
+// out[a][b] = sgn * acc (+ add[a][b] if add), row-major ni x ni
+template <int NB>
+__device__ __forceinline__ void blk_store(const f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni,
+                                          double *__restrict__ out, double sgn, const double *add) {
+    constexpr int TB = (NB + 1) / 2;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int wr = w >> 1, wc = w & 1, lg = l >> 4, lc = l & 15;
+#pragma unroll
+    for (int ti = 0; ti < TB; ti++) {
+        const int bi = wr + 2 * ti;
+        if (bi >= NB) continue;
+#pragma unroll
+        for (int tj = 0; tj < TB; tj++) {
+            const int bj = wc + 2 * tj, cc = 16 * bj + lc;
+            if (bj >= NB || cc >= ni) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * bi + 4 * r + lg;
+                if (i < ni) {
+                    const size_t e = (size_t)i * ni + cc;
+                    out[e] = add ? add[e] + sgn * acc[ti][tj][r] : sgn * acc[ti][tj][r];
+                }
+            }
+        }
+    }
+}
+
+// invert A[layer] for the listed layers (the eliminated ones of a level, or
+// the top layer of each column) into Dinv[layer] (transposed storage)
+template <int NB>
+__global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 *__restrict__ el,
+                                                       const int2 *__restrict__ top, const int *__restrict__ off,
+                                                       const double *__restrict__ A, double *__restrict__ Dinv,
+                                                       int *__restrict__ info, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int layer = el ? el[q].x : top[q].x, c = el ? el[q].w : top[q].y;
+    if (st && st[c].done) return;
+    __shared__ BlkShared<NB> sh;
+    if (threadIdx.x == 0) sh.info = 0;
+    const size_t nn = (size_t)ni * ni;
+    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    blk_load_schur<NB>(acc, ni, A + layer * nn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    blk_invert<NB>(acc, ni, Dinv + layer * nn, sh, layer - off[c] + 1);
+    if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
+}
+
+// level 0: couplings are the diagonals Lsub (B) and Usup (C); elementwise
+__global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const int *__restrict__ sflag,
+                              int slot0, const double *__restrict__ Lsub, const double *__restrict__ Usup,
+                              const double *__restrict__ Dinv, double *__restrict__ A, double *__restrict__ B,
+                              double *__restrict__ Cc, double *__restrict__ P, double *__restrict__ Q,
+                              const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const int fl = sflag[q];
+    const size_t nn = (size_t)ni * ni, j = e.x;
+    const size_t slot = (size_t)(slot0 + q) * nn;
+    const double *XL = e.y >= 0 ? Dinv + (size_t)e.y * nn : nullptr;   // X[m][i] = inv[i][m]
+    const double *XR = e.z >= 0 ? Dinv + (size_t)e.z * nn : nullptr;
+    for (int t = threadIdx.x; t < ni * ni; t += blockDim.x) {
+        const int a = t / ni, b = t % ni;
+        double upd = 0.0;
+        if (XL) {
+            const double p = Lsub[j * ni + a] * XL[(size_t)b * ni + a];
+            P[slot + t] = p;
+            upd = p * Usup[(size_t)e.y * ni + b];
+            B[j * nn + t] = (fl & 1) ? -(p * Lsub[(size_t)e.y * ni + b]) : 0.0;
+        } else {
+            B[j * nn + t] = 0.0;
+        }
+        if (XR) {
+            const double qv = Usup[j * ni + a] * XR[(size_t)b * ni + a];
+            Q[slot + t] = qv;
+            upd = upd + qv * Lsub[(size_t)e.z * ni + b];
+            Cc[j * nn + t] = (fl & 2) ? -(qv * Usup[(size_t)e.z * ni + b]) : 0.0;
+        } else {
+            Cc[j * nn + t] = 0.0;
+        }
+        A[j * nn + t] = A[j * nn + t] - upd;
+    }
+}
+
+// levels >= 1: dense couplings, products on MFMA; one workgroup per surviving layer
+template <int NB>
+__global__ void __launch_bounds__(NT, 2) k_bcr_update(int ni, int n, const int4 *__restrict__ sv,
+                                                       const int *__restrict__ sflag, int slot0,
+                                                       const double *__restrict__ Dinv, double *__restrict__ A,
+                                                       double *__restrict__ B, double *__restrict__ Cc,
+                                                       double *__restrict__ P, double *__restrict__ Q,
+                                                       const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const int fl = sflag[q];
+    const size_t nn = (size_t)ni * ni, j = e.x;
+    double *Pj = P + (size_t)(slot0 + q) * nn, *Qj = Q + (size_t)(slot0 + q) * nn;
+    __shared__ GemmShared<NB> g;
+    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    // P = B_j X_l, Q = C_j X_r (inverse stored transposed: inv(c, b) = X[b][c])
+    if (e.y >= 0) {
+        blk_zero<NB>(acc);
+        blk_gemm_acc<NB>(acc, ni, B + j * nn, ni, 1, Dinv + (size_t)e.y * nn, 1, ni, g);
+        blk_store<NB>(acc, ni, Pj, 1.0, nullptr);
+    }
+    if (e.z >= 0) {
+        blk_zero<NB>(acc);
+        blk_gemm_acc<NB>(acc, ni, Cc + j * nn, ni, 1, Dinv + (size_t)e.z * nn, 1, ni, g);
+        blk_store<NB>(acc, ni, Qj, 1.0, nullptr);
+    }
+    __threadfence_block();
+    __syncthreads();
+    // A_j -= P C_l + Q B_r
+    blk_zero<NB>(acc);
+    if (e.y >= 0) blk_gemm_acc<NB>(acc, ni, Pj, ni, 1, Cc + (size_t)e.y * nn, ni, 1, g);
+    if (e.z >= 0) blk_gemm_acc<NB>(acc, ni, Qj, ni, 1, B + (size_t)e.z * nn, ni, 1, g);
+    blk_store<NB>(acc, ni, A + j * nn, -1.0, A + j * nn);
+    // new couplings: B_j = -P B_l (to l's left neighbour), C_j = -Q C_r
+    blk_zero<NB>(acc);
+    if (fl & 1) blk_gemm_acc<NB>(acc, ni, Pj, ni, 1, B + (size_t)e.y * nn, ni, 1, g);
+    blk_store<NB>(acc, ni, B + j * nn, -1.0, nullptr);
+    blk_zero<NB>(acc);
+    if (fl & 2) blk_gemm_acc<NB>(acc, ni, Qj, ni, 1, Cc + (size_t)e.z * nn, ni, 1, g);
+    blk_store<NB>(acc, ni, Cc + j * nn, -1.0, nullptr);
+}
+
+__global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// solves: forward reduction r_j -= P_j r_l + Q_j r_r at level s
+__global__ void k_bcr_fwd(int ni, int n, const int4 *__restrict__ sv, int slot0, const double *__restrict__ P,
+                          const double *__restrict__ Q, double *__restrict__ r, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const size_t nn = (size_t)ni * ni;
+    const double *Pj = P + (size_t)(slot0 + q) * nn, *Qj = Q + (size_t)(slot0 + q) * nn;
+    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
+        rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double v = r[(size_t)e.x * ni + a];
+        if (e.y >= 0) {
+            double s = 0.0;
+            for (int b = 0; b < ni; b++) s = fma(Pj[(size_t)a * ni + b], rl[b], s);
+            v = v - s;
+        }
+        if (e.z >= 0) {
+            double s = 0.0;
+            for (int b = 0; b < ni; b++) s = fma(Qj[(size_t)a * ni + b], rr[b], s);
+            v = v - s;
+        }
+        r[(size_t)e.x * ni + a] = v;
+    }
+}
+
+// x_k = A_k^-1 (r_k - B_k x_l - C_k x_r) for the eliminated layers of level s
+// (level 0: diagonal couplings Lsub, Usup); top: x = A^-1 r
+__global__ void k_bcr_bwd(int ni, int n, const int4 *__restrict__ el, const int2 *__restrict__ top, int level,
+                          const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                          const double *__restrict__ Usup, const double *__restrict__ B, const double *__restrict__ Cc,
+                          const double *__restrict__ r, double *__restrict__ x, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    int k, left = -1, right = -1, c;
+    if (el) { const int4 e = el[q]; k = e.x; left = e.y; right = e.z; c = e.w; }
+    else { k = top[q].x; c = top[q].y; }
+    if (st && st[c].done) return;
+    const size_t nn = (size_t)ni * ni;
+    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
+        xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double v = r[(size_t)k * ni + a];
+        if (level == 0) {
+            if (left >= 0) v = v - Lsub[(size_t)k * ni + a] * xl[a];
+            if (right >= 0) v = v - Usup[(size_t)k * ni + a] * xr[a];
+        } else {
+            if (left >= 0) {
+                double s = 0.0;
+                for (int b = 0; b < ni; b++) s = fma(B[(size_t)k * nn + (size_t)a * ni + b], xl[b], s);
+                v = v - s;
+            }
+            if (right >= 0) {
+                double s = 0.0;
+                for (int b = 0; b < ni; b++) s = fma(Cc[(size_t)k * nn + (size_t)a * ni + b], xr[b], s);
+                v = v - s;
+            }
+        }
+        w[a] = v;
+    }
+    __syncthreads();
+    const double *X = Dinv + (size_t)k * nn;   // X[m][a] = inv[a][m]
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double s = 0.0;
+        for (int m = 0; m < ni; m++) s = fma(X[(size_t)m * ni + a], w[m], s);
+        x[(size_t)k * ni + a] = s;
+    }
+}
+
 // D = A B for A[i][k] = i + 0.25 k + 1, B[k][j] = 1 - 0.5 k + 0.125 j (16 x 4 x 16)
 // through one v_mfma_f64_16x16x4f64 with the operand layout the blocked
 // inversion assumes; out[i*16+j] = D[i][j].
@@ -1113,13 +1392,83 @@ void launch_factor_n(int variant, int ni, int ncols, const int *off, const doubl
 
 }  // namespace
 
+namespace {
+template <int NB>
+int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const double *Lsub, const double *Usup,
+                        double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
+    const vk_bcr_sched &b = cols->bcr;
+    const long nd = (long)cols->L * ni * ni;
+    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, nd, D, ws->A);
+    for (int lev = 0; lev < b.nlev; lev++) {
+        const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
+        const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
+        if (ne > 0)
+            hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(ne), dim3(NT), 0, s, ni, ne, b.d_elim + e0,
+                               (const int2 *)nullptr, cols->d_off, ws->A, Dinv, info, st);
+        if (ns > 0) {
+            if (lev == 0)
+                hipLaunchKernelGGL(k_bcr_update0, dim3(ns), dim3(NT), 0, s, ni, ns, b.d_surv + s0, b.d_sflag + s0, s0,
+                                   Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+            else
+                hipLaunchKernelGGL((k_bcr_update<NB>), dim3(ns), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
+                                   b.d_sflag + s0, s0, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+        }
+    }
+    hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(b.ntop), dim3(NT), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top,
+                       cols->d_off, ws->A, Dinv, info, st);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, const double *Lsub, const double *Usup,
+                     const double *in, double *out, double *tmp, const vk_colstate *st, const vk_bcr_ws *ws,
+                     hipStream_t s) {
+    const vk_bcr_sched &b = cols->bcr;
+    const long n = (long)cols->L * ni;
+    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
+    const int bt = ni <= 64 ? 64 : (ni <= 128 ? 128 : 192);
+    for (int lev = 0; lev < b.nlev; lev++) {
+        const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
+        if (ns > 0)
+            hipLaunchKernelGGL(k_bcr_fwd, dim3(ns), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
+    }
+    hipLaunchKernelGGL(k_bcr_bwd, dim3(b.ntop), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, 0, Dinv,
+                       Lsub, Usup, ws->B, ws->C, tmp, out, st);
+    for (int lev = b.nlev - 1; lev >= 0; lev--) {
+        const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
+        if (ne > 0)
+            hipLaunchKernelGGL(k_bcr_bwd, dim3(ne), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, lev,
+                               Dinv, Lsub, Usup, ws->B, ws->C, tmp, out, st);
+    }
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+}  // namespace
+
 int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *D, const double *Lsub,
-                     const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
+                     const double *Usup, double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws,
+                     hipStream_t s) {
     const int ni = plan->ni;
     const int n = (ni + 15) / 16;
     const int nc = cols->ncols;
     const int v = plan->factor_variant;
     if (info) hipLaunchKernelGGL(k_zero_int, dim3((nc + 255) / 256), dim3(256), 0, s, nc, info);
+    if (v == VK_FACTOR_BCR) {
+        if (!ws || !ws->A) return VK_EINVAL;
+        switch (n) {
+        case 1: return launch_bcr_factor_n<1>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 2: return launch_bcr_factor_n<2>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 3: return launch_bcr_factor_n<3>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 4: return launch_bcr_factor_n<4>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 5: return launch_bcr_factor_n<5>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 6: return launch_bcr_factor_n<6>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 7: return launch_bcr_factor_n<7>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 8: return launch_bcr_factor_n<8>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 9: return launch_bcr_factor_n<9>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        case 10: return launch_bcr_factor_n<10>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+        default: return VK_ETOOBIG;
+        }
+    }
     switch (n) {
     case 1: launch_factor_n<1>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
     case 2: launch_factor_n<2>(v, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s); break;
@@ -1149,9 +1498,13 @@ void launch_solve_n(int ni, int ncols, const int *off, const double *Dinv, const
 
 int vk_launch_solve(const vk_plan *plan, const vk_columns *cols, const double *Dinv, const double *Lsub,
                     const double *Usup, const double *in, double *out, double *tmp, const vk_colstate *st,
-                    hipStream_t s) {
+                    const vk_bcr_ws *ws, hipStream_t s) {
     const int ni = plan->ni, nc = cols->ncols;
     const int *off = cols->d_off;
+    if (plan->factor_variant == VK_FACTOR_BCR) {
+        if (!ws || !ws->A) return VK_EINVAL;
+        return launch_bcr_solve(ni, cols, Dinv, Lsub, Usup, in, out, tmp, st, ws, s);
+    }
     switch ((ni + 15) / 16) {
     case 1: launch_solve_n<1>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
     case 2: launch_solve_n<2>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
@@ -1183,14 +1536,14 @@ int vk_launch_residual(const vk_plan *plan, const vk_columns *cols, const double
 // x = A^-1 b with `refine` steps of iterative refinement (work, tmp: L*ni scratch each)
 int vk_launch_solve_refined(const vk_plan *plan, const vk_columns *cols, const double *D, const double *Dinv,
                             const double *Lsub, const double *Usup, const double *b, double *x, double *work,
-                            double *tmp, int refine, const vk_colstate *st, hipStream_t s) {
-    int r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, b, x, tmp, st, s);
+                            double *tmp, int refine, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
+    int r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, b, x, tmp, st, ws, s);
     if (r) return r;
     const int n = cols->L * plan->ni;
     for (int q = 0; q < refine; q++) {
         r = vk_launch_residual(plan, cols, D, Lsub, Usup, x, b, work, st, s);
         if (r) return r;
-        r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, work, work, tmp, st, s);
+        r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, work, work, tmp, st, ws, s);
         if (r) return r;
         hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, n, plan->ni, cols->d_layer_col, work, x,
                            st);
@@ -1199,20 +1552,66 @@ int vk_launch_solve_refined(const vk_plan *plan, const vk_columns *cols, const d
     return 0;
 }
 
+// stream-ordered workspace of the cyclic reduction (freed with the object)
+struct BcrTmp {
+    vk_bcr_ws ws;
+    hipStream_t s;
+    explicit BcrTmp(hipStream_t st) : s(st) {}
+    int alloc(const vk_plan *plan, const vk_columns *cols) {
+        const size_t nn = (size_t)plan->ni * plan->ni, L = (size_t)cols->L, ns = (size_t)cols->bcr.nsurv + 1;
+        if (hipMallocAsync((void **)&ws.A, sizeof(double) * L * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.B, sizeof(double) * L * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.C, sizeof(double) * L * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.P, sizeof(double) * ns * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.Q, sizeof(double) * ns * nn, s) != hipSuccess)
+            return VK_ENOMEM;
+        return 0;
+    }
+    ~BcrTmp() {
+        for (double *p : {ws.A, ws.B, ws.C, ws.P, ws.Q})
+            if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+extern "C" int vk_block_solve_system(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                                     const double *d_Lsub, const double *d_Usup, const double *d_b, double *d_x,
+                                     int refine, int *d_info, void *stream) {
+    if (!plan || !cols || !d_D || !d_Lsub || !d_Usup || !d_b || !d_x || refine < 0 || d_b == d_x) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)plan->ni * plan->ni, L = (size_t)cols->L;
+    double *Dinv = nullptr, *work = nullptr, *tmp = nullptr;
+    BcrTmp bt(s);
+    int r = 0;
+    if (hipMallocAsync((void **)&Dinv, sizeof(double) * L * nn, s) != hipSuccess ||
+        hipMallocAsync((void **)&work, sizeof(double) * L * plan->ni, s) != hipSuccess ||
+        hipMallocAsync((void **)&tmp, sizeof(double) * L * plan->ni, s) != hipSuccess)
+        r = VK_ENOMEM;
+    if (!r && plan->factor_variant == VK_FACTOR_BCR) r = bt.alloc(plan, cols);
+    if (!r) r = vk_launch_factor(plan, cols, d_D, d_Lsub, d_Usup, Dinv, d_info, nullptr, &bt.ws, s);
+    if (!r)
+        r = vk_launch_solve_refined(plan, cols, d_D, Dinv, d_Lsub, d_Usup, d_b, d_x, work, tmp, refine, nullptr,
+                                    &bt.ws, s);
+    for (double *p : {Dinv, work, tmp})
+        if (p) (void)hipFreeAsync(p, s);
+    return r;
+}
+
 extern "C" int vk_block_factor(const vk_plan *plan, const vk_columns *cols, const double *d_D,
                                const double *d_Lsub, const double *d_Usup, double *d_Dinv, int *d_info,
                                void *stream) {
     if (!plan || !cols || !d_D || !d_Lsub || !d_Usup || !d_Dinv || d_D == d_Dinv) return VK_EINVAL;
-    return vk_launch_factor(plan, cols, d_D, d_Lsub, d_Usup, d_Dinv, d_info, nullptr, (hipStream_t)stream);
+    if (plan->factor_variant == VK_FACTOR_BCR) return VK_EINVAL;
+    return vk_launch_factor(plan, cols, d_D, d_Lsub, d_Usup, d_Dinv, d_info, nullptr, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int vk_block_solve(const vk_plan *plan, const vk_columns *cols, const double *d_Dinv,
                               const double *d_Lsub, const double *d_Usup, double *d_rhs, void *stream) {
     if (!plan || !cols || !d_Dinv || !d_Lsub || !d_Usup || !d_rhs) return VK_EINVAL;
+    if (plan->factor_variant == VK_FACTOR_BCR) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     double *tmp = nullptr;
     VK_CHECK(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)cols->L * plan->ni, s));
-    int r = vk_launch_solve(plan, cols, d_Dinv, d_Lsub, d_Usup, d_rhs, d_rhs, tmp, nullptr, s);
+    int r = vk_launch_solve(plan, cols, d_Dinv, d_Lsub, d_Usup, d_rhs, d_rhs, tmp, nullptr, nullptr, s);
     VK_CHECK(hipFreeAsync(tmp, s));
     return r;
 }
@@ -1245,6 +1644,9 @@ static bool variant_ok(const vk_plan *plan, int v) {
     return v >= 0 && v < VK_FACTOR_NVARIANTS && (v != VK_FACTOR_WAVE || plan->ni <= 112);
 }
 
+// cyclic reduction pays off while the columns leave most of the device idle
+static constexpr int BCR_TUNE_MAX_COLS = 64;
+
 extern "C" int vk_plan_set_factor_variant(vk_plan *plan, int variant) {
     if (!plan || !variant_ok(plan, variant)) return VK_EINVAL;
     plan->factor_variant = variant;
@@ -1257,14 +1659,19 @@ extern "C" int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, f
     if (!plan || !cols || reps < 1 || !ms) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     const size_t nn = (size_t)plan->ni * plan->ni, L = (size_t)cols->L;
-    double *D = nullptr, *Dinv = nullptr, *Lsub = nullptr, *Usup = nullptr;
+    const bool try_bcr = cols->ncols <= BCR_TUNE_MAX_COLS;
+    double *D = nullptr, *Dinv = nullptr, *Lsub = nullptr, *Usup = nullptr, *x = nullptr, *tmp = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    BcrTmp bt(s);
     int rc = 0;
     if (hipMallocAsync((void **)&D, sizeof(double) * L * nn, s) != hipSuccess ||
         hipMallocAsync((void **)&Dinv, sizeof(double) * L * nn, s) != hipSuccess ||
         hipMallocAsync((void **)&Lsub, sizeof(double) * L * plan->ni, s) != hipSuccess ||
         hipMallocAsync((void **)&Usup, sizeof(double) * L * plan->ni, s) != hipSuccess ||
-        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        hipMallocAsync((void **)&x, sizeof(double) * L * plan->ni, s) != hipSuccess ||
+        hipMallocAsync((void **)&tmp, sizeof(double) * L * plan->ni, s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        (try_bcr && bt.alloc(plan, cols))) {
         rc = VK_ENOMEM;
     } else {
         const long nd = (long)(L * nn);
@@ -1274,15 +1681,22 @@ extern "C" int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, f
         int bestv = keep;
         float bestms = 0.f;
         bool have = false;
+        // one repetition: a factorisation and the two solves of an attempt
+        auto rep = [&]() -> int {
+            int r = vk_launch_factor(plan, cols, D, Lsub, Usup, Dinv, nullptr, nullptr, &bt.ws, s);
+            for (int q = 0; q < 2 && !r; q++) r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, Lsub, x, tmp, nullptr,
+                                                                  &bt.ws, s);
+            return r;
+        };
         for (int v = 0; v < VK_FACTOR_NVARIANTS && rc == 0; v++) {
             ms[v] = -1.0f;
-            if (!variant_ok(plan, v)) continue;
+            if (!variant_ok(plan, v) || (v == VK_FACTOR_BCR && !try_bcr)) continue;
             plan->factor_variant = v;
-            rc = vk_launch_factor(plan, cols, D, Lsub, Usup, Dinv, nullptr, nullptr, s);   // warm-up
+            rc = rep();   // warm-up
             if (rc) break;
             hipError_t e = hipEventRecord(e0, s);
             if (e != hipSuccess) { rc = -(int)e; break; }
-            for (int q = 0; q < reps && rc == 0; q++) rc = vk_launch_factor(plan, cols, D, Lsub, Usup, Dinv, nullptr, nullptr, s);
+            for (int q = 0; q < reps && rc == 0; q++) rc = rep();
             if (rc) break;
             e = hipEventRecord(e1, s);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1294,10 +1708,8 @@ extern "C" int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, f
         }
         plan->factor_variant = rc ? keep : bestv;
     }
-    if (D) (void)hipFreeAsync(D, s);
-    if (Dinv) (void)hipFreeAsync(Dinv, s);
-    if (Lsub) (void)hipFreeAsync(Lsub, s);
-    if (Usup) (void)hipFreeAsync(Usup, s);
+    for (double *p : {D, Dinv, Lsub, Usup, x, tmp})
+        if (p) (void)hipFreeAsync(p, s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     return rc;

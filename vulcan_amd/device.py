@@ -26,7 +26,7 @@ def _require_device():
         raise _lib.VulcanAmdError('no HIP device visible: the vulcan_amd kernels need an MI355X (gfx950)')
 
 
-FACTOR_NVARIANTS = 4     # VK_FACTOR_NVARIANTS
+FACTOR_NVARIANTS = 5     # VK_FACTOR_NVARIANTS
 
 
 def stream_handle(stream: Optional[torch.cuda.Stream] = None) -> C.c_void_p:

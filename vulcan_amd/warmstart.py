@@ -176,14 +176,14 @@ class RunResult:
 
 
 def run_to_steady(net: Network, cases: Sequence[ColumnCase], max_attempts: int = 20000, poll: int = 16,
-                  plan=None, log=None) -> RunResult:
+                  plan=None, log=None, tune: bool = True) -> RunResult:
     """All columns in one batch; the device loop stops a column at conv()
     (or count_max / runtime); the host polls every `poll` attempts."""
     import torch
     from .device import ColumnBatch, DevicePlan
     plan = plan or DevicePlan(net)
     cfg = cases[0].cfg
-    b = ColumnBatch(plan, [c.col for c in cases], [c.y0 for c in cases], cfg, history=cfg.conv_step + 1)
+    b = ColumnBatch(plan, [c.col for c in cases], [c.y0 for c in cases], cfg, history=cfg.conv_step + 1, tune=tune)
     nc = len(cases)
     wall = np.full(nc, np.nan)
     torch.cuda.synchronize()
