@@ -9,12 +9,13 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
-SOURCES = ['vk_chem.hip', 'vk_solve.hip', 'vk_ros2.hip', 'vk_mlp.hip']
+SOURCES = ['vk_solve.hip', 'vk_factor_reg.hip', 'vk_factor_blk1.hip', 'vk_factor_blk2.hip', 'vk_factor_wave.hip', 'vk_bcr.hip',
+           'vk_chem.hip', 'vk_ros2.hip', 'vk_mlp.hip']
 TARGET = os.path.join(PKG, 'libvulcan_amd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
 FLAGS = [
-    '-O3', '--offload-arch=gfx950', '-fPIC', '-shared', '-std=c++17',
+    '-O3', '--offload-arch=gfx950', '-fPIC', '-std=c++17',
     # the kernels follow the reference's expression trees operation by
     # operation; fused multiply-adds are written explicitly where wanted
     '-ffp-contract=off',
@@ -24,15 +25,33 @@ FLAGS = [
 
 
 def build(verbose=False):
-    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    """Compiles each source to an object in parallel (the translation units
+    are independent: no relocatable device code), then links the library."""
+    objdir = os.path.join(HERE, '_build')
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src.replace('.hip', '.o'))
+        cmd = [HIPCC] + FLAGS + ['-c', os.path.join(HERE, src), '-o', obj]
+        if verbose:
+            print(' '.join(cmd), flush=True)
+        procs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for src, obj, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError('hipcc failed compiling ' + ', '.join(failed))
     tmp = TARGET + '.tmp'
-    cmd = [HIPCC] + FLAGS + srcs + ['-o', tmp]
+    cmd = [HIPCC, '--offload-arch=gfx950', '-fPIC', '-shared'] + [o for _, o, _ in procs] + ['-o', tmp]
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError('hipcc failed building libvulcan_amd.so')
+        raise RuntimeError('hipcc failed linking libvulcan_amd.so')
     os.replace(tmp, TARGET)
     return TARGET
 

@@ -1,0 +1,282 @@
+// Block cyclic reduction factorisation and solves (factor variant 4).
+#include "vk_blk.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Block cyclic reduction (factor variant 4)
+//
+// Every level eliminates the odd-positioned active layers of each column at
+// once: their blocks are inverted in parallel (blk_invert), and each
+// surviving layer j with eliminated neighbours l (left) and r (right) takes
+//     P_j = B_j A_l^-1,   Q_j = C_j A_r^-1,
+//     A_j -= P_j C_l + Q_j B_r,   B_j = -P_j B_l,   C_j = -Q_j C_r
+// (B: coupling to the previous active layer, C: to the next; diagonal at
+// level 0, dense above).  The chain over layers becomes log2(nz) levels of
+// independent work: the Schur-complement updates above level 0 are dense
+// n x n products on FP64 MFMA, level 0 is elementwise.  P and Q are kept
+// for the right-hand side reduction of the solves.
+//
+// Storage: A row-major (it is inverted in place of D); P, Q, B, C
+// transposed (Pt[b][a] = P[a][b]), so that the solve sweeps, one thread per
+// row a, read them coalesced; inverses transposed as in the chain variants.
+// ---------------------------------------------------------------------------
+
+// invert A[layer] for the listed layers (the eliminated ones of a level, or
+// the top layer of each column) into Dinv[layer] (transposed storage)
+template <int NB>
+__global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 *__restrict__ el,
+                                                       const int2 *__restrict__ top, const int *__restrict__ off,
+                                                       const double *__restrict__ A, double *__restrict__ Dinv,
+                                                       int *__restrict__ info, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int layer = el ? el[q].x : top[q].x, c = el ? el[q].w : top[q].y;
+    if (st && st[c].done) return;
+    __shared__ BlkShared<NB> sh;
+    if (threadIdx.x == 0) sh.info = 0;
+    const size_t nn = (size_t)ni * ni;
+    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    blk_load_schur<NB>(acc, ni, A + layer * nn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    blk_invert<NB>(acc, ni, Dinv + layer * nn, sh, layer - off[c] + 1);
+    if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
+}
+
+// level 0: couplings are the diagonals Lsub (B) and Usup (C); elementwise,
+// over the transposed index tt = b*ni + a (coalesced P, Q, B, C and inverses)
+__global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const int *__restrict__ sflag,
+                              int slot0, const double *__restrict__ Lsub, const double *__restrict__ Usup,
+                              const double *__restrict__ Dinv, double *__restrict__ A, double *__restrict__ Bt,
+                              double *__restrict__ Ct, double *__restrict__ Pt, double *__restrict__ Qt,
+                              const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const int fl = sflag[q];
+    const size_t nn = (size_t)ni * ni, j = e.x;
+    const size_t slot = (size_t)(slot0 + q) * nn;
+    const double *XL = e.y >= 0 ? Dinv + (size_t)e.y * nn : nullptr;   // X[m][i] = inv[i][m]
+    const double *XR = e.z >= 0 ? Dinv + (size_t)e.z * nn : nullptr;
+    for (int tt = threadIdx.x; tt < ni * ni; tt += blockDim.x) {
+        const int b = tt / ni, a = tt - b * ni;
+        double upd = 0.0;
+        if (XL) {
+            const double p = Lsub[j * ni + a] * XL[tt];
+            Pt[slot + tt] = p;
+            upd = p * Usup[(size_t)e.y * ni + b];
+            Bt[j * nn + tt] = (fl & 1) ? -(p * Lsub[(size_t)e.y * ni + b]) : 0.0;
+        } else {
+            Bt[j * nn + tt] = 0.0;
+        }
+        if (XR) {
+            const double qv = Usup[j * ni + a] * XR[tt];
+            Qt[slot + tt] = qv;
+            upd = upd + qv * Lsub[(size_t)e.z * ni + b];
+            Ct[j * nn + tt] = (fl & 2) ? -(qv * Usup[(size_t)e.z * ni + b]) : 0.0;
+        } else {
+            Ct[j * nn + tt] = 0.0;
+        }
+        A[j * nn + (size_t)a * ni + b] = A[j * nn + (size_t)a * ni + b] - upd;
+    }
+}
+
+// levels >= 1: dense couplings, products on MFMA; one workgroup per
+// (surviving layer, 32-row slab).  A slab's output rows depend only on the
+// same rows of the left operands, so the slabs are independent.
+template <int NB>
+__global__ void __launch_bounds__(NT) k_bcr_update(int ni, int n, const int4 *__restrict__ sv,
+                                                   const int *__restrict__ sflag, int slot0,
+                                                   const double *__restrict__ Dinv, double *__restrict__ A,
+                                                   double *__restrict__ Bt, double *__restrict__ Ct,
+                                                   double *__restrict__ Pt, double *__restrict__ Qt,
+                                                   const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x, r0 = 32 * blockIdx.y;
+    if (q >= n || r0 >= ni) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const int fl = sflag[q];
+    const size_t nn = (size_t)ni * ni, j = e.x;
+    double *Ptj = Pt + (size_t)(slot0 + q) * nn, *Qtj = Qt + (size_t)(slot0 + q) * nn;
+    __shared__ SlabShared<NB> g;
+    f64x4 acc[(NB + 1) / 2];
+    // P = B_j X_l, Q = C_j X_r: B_j(a, c) = Bt_j[c][a], inv(c, b) = X[b][c]
+    if (e.y >= 0) {
+        slab_zero<NB>(acc);
+        slab_gemm<NB>(acc, ni, r0, Bt + j * nn, 1, ni, Dinv + (size_t)e.y * nn, 1, ni, g);
+        slab_store<NB>(acc, ni, r0, Ptj, true, 1.0, nullptr);
+    }
+    if (e.z >= 0) {
+        slab_zero<NB>(acc);
+        slab_gemm<NB>(acc, ni, r0, Ct + j * nn, 1, ni, Dinv + (size_t)e.z * nn, 1, ni, g);
+        slab_store<NB>(acc, ni, r0, Qtj, true, 1.0, nullptr);
+    }
+    __threadfence_block();
+    __syncthreads();
+    // A_j -= P C_l + Q B_r
+    slab_zero<NB>(acc);
+    if (e.y >= 0) slab_gemm<NB>(acc, ni, r0, Ptj, 1, ni, Ct + (size_t)e.y * nn, 1, ni, g);
+    if (e.z >= 0) slab_gemm<NB>(acc, ni, r0, Qtj, 1, ni, Bt + (size_t)e.z * nn, 1, ni, g);
+    slab_store<NB>(acc, ni, r0, A + j * nn, false, -1.0, A + j * nn);
+    // new couplings: B_j = -P B_l (to l's left neighbour), C_j = -Q C_r
+    slab_zero<NB>(acc);
+    if (fl & 1) slab_gemm<NB>(acc, ni, r0, Ptj, 1, ni, Bt + (size_t)e.y * nn, 1, ni, g);
+    slab_store<NB>(acc, ni, r0, Bt + j * nn, true, -1.0, nullptr);
+    slab_zero<NB>(acc);
+    if (fl & 2) slab_gemm<NB>(acc, ni, r0, Qtj, 1, ni, Ct + (size_t)e.z * nn, 1, ni, g);
+    slab_store<NB>(acc, ni, r0, Ct + j * nn, true, -1.0, nullptr);
+}
+
+__global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// solves: forward reduction r_j -= P_j r_l + Q_j r_r at level s
+__global__ void k_bcr_fwd(int ni, int n, const int4 *__restrict__ sv, int slot0, const double *__restrict__ P,
+                          const double *__restrict__ Q, double *__restrict__ r, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const size_t nn = (size_t)ni * ni;
+    const double *Pj = P + (size_t)(slot0 + q) * nn, *Qj = Q + (size_t)(slot0 + q) * nn;
+    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
+        rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double v = r[(size_t)e.x * ni + a];
+        if (e.y >= 0) {
+            double s = 0.0;
+            for (int b = 0; b < ni; b++) s = fma(Pj[(size_t)b * ni + a], rl[b], s);
+            v = v - s;
+        }
+        if (e.z >= 0) {
+            double s = 0.0;
+            for (int b = 0; b < ni; b++) s = fma(Qj[(size_t)b * ni + a], rr[b], s);
+            v = v - s;
+        }
+        r[(size_t)e.x * ni + a] = v;
+    }
+}
+
+// x_k = A_k^-1 (r_k - B_k x_l - C_k x_r) for the eliminated layers of level s
+// (level 0: diagonal couplings Lsub, Usup); top: x = A^-1 r
+__global__ void k_bcr_bwd(int ni, int n, const int4 *__restrict__ el, const int2 *__restrict__ top, int level,
+                          const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                          const double *__restrict__ Usup, const double *__restrict__ B, const double *__restrict__ Cc,
+                          const double *__restrict__ r, double *__restrict__ x, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    int k, left = -1, right = -1, c;
+    if (el) { const int4 e = el[q]; k = e.x; left = e.y; right = e.z; c = e.w; }
+    else { k = top[q].x; c = top[q].y; }
+    if (st && st[c].done) return;
+    const size_t nn = (size_t)ni * ni;
+    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
+        xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double v = r[(size_t)k * ni + a];
+        if (level == 0) {
+            if (left >= 0) v = v - Lsub[(size_t)k * ni + a] * xl[a];
+            if (right >= 0) v = v - Usup[(size_t)k * ni + a] * xr[a];
+        } else {
+            if (left >= 0) {
+                double s = 0.0;
+                for (int b = 0; b < ni; b++) s = fma(B[(size_t)k * nn + (size_t)b * ni + a], xl[b], s);
+                v = v - s;
+            }
+            if (right >= 0) {
+                double s = 0.0;
+                for (int b = 0; b < ni; b++) s = fma(Cc[(size_t)k * nn + (size_t)b * ni + a], xr[b], s);
+                v = v - s;
+            }
+        }
+        w[a] = v;
+    }
+    __syncthreads();
+    const double *X = Dinv + (size_t)k * nn;   // X[m][a] = inv[a][m]
+    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+        double s = 0.0;
+        for (int m = 0; m < ni; m++) s = fma(X[(size_t)m * ni + a], w[m], s);
+        x[(size_t)k * ni + a] = s;
+    }
+}
+
+template <int NB>
+int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const double *Lsub, const double *Usup,
+                        double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
+    const vk_bcr_sched &b = cols->bcr;
+    const long nd = (long)cols->L * ni * ni;
+    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, nd, D, ws->A);
+    for (int lev = 0; lev < b.nlev; lev++) {
+        const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
+        const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
+        if (ne > 0)
+            hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(ne), dim3(NT), 0, s, ni, ne, b.d_elim + e0,
+                               (const int2 *)nullptr, cols->d_off, ws->A, Dinv, info, st);
+        if (ns > 0) {
+            if (lev == 0)
+                hipLaunchKernelGGL(k_bcr_update0, dim3(ns), dim3(NT), 0, s, ni, ns, b.d_surv + s0, b.d_sflag + s0, s0,
+                                   Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+            else
+                hipLaunchKernelGGL((k_bcr_update<NB>), dim3(ns, (ni + 31) / 32), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
+                                   b.d_sflag + s0, s0, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+        }
+    }
+    hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(b.ntop), dim3(NT), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top,
+                       cols->d_off, ws->A, Dinv, info, st);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+int vk_launch_bcr_factor(int nb, int ni, const vk_columns *cols, const double *D, const double *Lsub,
+                         const double *Usup, double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws,
+                         hipStream_t s) {
+    switch (nb) {
+    case 1: return launch_bcr_factor_n<1>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 2: return launch_bcr_factor_n<2>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 3: return launch_bcr_factor_n<3>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 4: return launch_bcr_factor_n<4>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 5: return launch_bcr_factor_n<5>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 6: return launch_bcr_factor_n<6>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 7: return launch_bcr_factor_n<7>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 8: return launch_bcr_factor_n<8>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 9: return launch_bcr_factor_n<9>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    case 10: return launch_bcr_factor_n<10>(ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
+    default: return VK_ETOOBIG;
+    }
+}
+
+int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, const double *Lsub, const double *Usup,
+                        const double *in, double *out, double *tmp, const vk_colstate *st, const vk_bcr_ws *ws,
+                        hipStream_t s) {
+    const vk_bcr_sched &b = cols->bcr;
+    const long n = (long)cols->L * ni;
+    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
+    const int bt = ni <= 64 ? 64 : (ni <= 128 ? 128 : 192);
+    for (int lev = 0; lev < b.nlev; lev++) {
+        const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
+        if (ns > 0)
+            hipLaunchKernelGGL(k_bcr_fwd, dim3(ns), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
+    }
+    hipLaunchKernelGGL(k_bcr_bwd, dim3(b.ntop), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, 0, Dinv,
+                       Lsub, Usup, ws->B, ws->C, tmp, out, st);
+    for (int lev = b.nlev - 1; lev >= 0; lev--) {
+        const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
+        if (ne > 0)
+            hipLaunchKernelGGL(k_bcr_bwd, dim3(ne), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, lev,
+                               Dinv, Lsub, Usup, ws->B, ws->C, tmp, out, st);
+    }
+    VK_CHECK(hipGetLastError());
+    return 0;
+}

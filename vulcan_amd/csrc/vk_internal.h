@@ -92,3 +92,20 @@ struct vk_integrator {
     std::vector<char> gkey;    // arguments baked into gexec
     std::vector<void *> allocs;
 };
+
+// Block-inversion launchers of the factor variants (nb = ceil(ni / 16),
+// one translation unit each); 0 or a VK_E* code.
+int vk_launch_factor_register(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
+int vk_launch_factor_blocked1(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
+int vk_launch_factor_blocked2(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
+int vk_launch_factor_wave(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+                          const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
+int vk_launch_bcr_factor(int nb, int ni, const vk_columns *cols, const double *D, const double *Lsub,
+                         const double *Usup, double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws,
+                         hipStream_t s);
+int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, const double *Lsub, const double *Usup,
+                        const double *in, double *out, double *tmp, const vk_colstate *st, const vk_bcr_ws *ws,
+                        hipStream_t s);
