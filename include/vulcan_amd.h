@@ -202,8 +202,9 @@ int vk_block_solve_system(const vk_plan *plan, const vk_columns *cols, const dou
  * B[k][j] = 1 - k/2 + j/8 (16 x 4 x 16), computed by one MFMA. */
 int vk_selftest_mfma_f64(double *d_out, void *stream);
 /* Self-test of the wave-wide maximum used by the blocked inversion's pivot
- * search: out[l] = max over the 64 lanes of in[], for every lane l. */
-int vk_selftest_wave_max_u64(const unsigned long long *d_in, unsigned long long *d_out, void *stream);
+ * search (32-bit keys): out[l] = max over the 64 lanes of in[], for every
+ * lane l. */
+int vk_selftest_wave_max_u32(const unsigned *d_in, unsigned *d_out, void *stream);
 
 /* replaces the gbtrs half of solve_banded (op.py:2914, 2929): rhs <- lhs^-1 rhs */
 int vk_block_solve(const vk_plan *plan, const vk_columns *cols, const double *d_Dinv,

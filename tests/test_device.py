@@ -164,22 +164,22 @@ def test_mfma_f64_operand_layout():
 
 
 @gpu
-def test_wave_max_u64():
+def test_wave_max_u32():
     """The DPP + readlane wave maximum of the blocked pivot search."""
     torch = _torch()
     from vulcan_amd.device import dptr, stream_handle
     rng = np.random.default_rng(3)
     for trial in range(4):
-        v = rng.integers(0, 2 ** 62, 64, dtype=np.int64)
+        v = rng.integers(0, 2 ** 31, 64, dtype=np.int64).astype(np.uint32)
         if trial == 1:
             v[:] = 0; v[37] = 5
         if trial == 2:
-            v[63] = 2 ** 62 + 7
-        vin = torch.as_tensor(v, device='cuda')
+            v[63] = 2 ** 32 - 7                 # top bit set: compared unsigned
+        vin = torch.as_tensor(v.view(np.int32), device='cuda')
         out = torch.zeros_like(vin)
-        _lib.check(_lib.load().vk_selftest_wave_max_u64(dptr(vin), dptr(out), stream_handle()),
-                   'vk_selftest_wave_max_u64')
-        assert (out.cpu().numpy() == v.max()).all(), trial
+        _lib.check(_lib.load().vk_selftest_wave_max_u32(dptr(vin), dptr(out), stream_handle()),
+                   'vk_selftest_wave_max_u32')
+        assert (out.cpu().numpy().view(np.uint32) == v.max()).all(), trial
 
 
 FACTOR_VARIANTS = [0, 1, 2, 3, 4]   # register GJ, blocked MFMA (1 or 2 WGs per CU), two-wave, cyclic reduction

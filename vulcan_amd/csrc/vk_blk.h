@@ -11,15 +11,19 @@ constexpr int NT = 256;
 // ---------------------------------------------------------------------------
 // Blocked Gauss-Jordan inversion on FP64 MFMA (factor variant 1)
 //
-// The pivot steps are grouped into panels of 16 columns.  The 16 pivot
-// steps of a panel run on the panel columns alone, spread over all four
-// waves (a thread per (column, row group)), with two barriers per step:
-// one to combine the per-wave pivot keys (|value| bits with the row in the
-// low byte), one to publish the pivot row.  The other columns then receive
-// the whole panel at once, as the rank-16 Schur update
+// The pivot steps are grouped into panels of 16 columns.  One wave runs the
+// 16 pivot steps of a panel on the panel columns alone (rows spread over the
+// lanes, pivot search by a wave-wide max over 32-bit (|value|, row) keys,
+// pivot row handed over through LDS: no workgroup barrier inside a panel;
+// the panel wave rotates with the panel and the workgroup index, so that
+// co-resident workgroups do not run their panels on the same SIMD; a version
+// spreading the panel over four waves, two barriers per step, measured
+// slower).  The other columns then receive the whole panel at once, as the
+// rank-16 Schur update
 //     A[i][j] = (i in P ? 0 : A[i][j]) + sum_s Pan[i][s] A_old[p_s][j]
 // (P: the panel's pivot rows p_s, Pan: the factored panel columns), which is
-// a 16-deep GEMM and runs on v_mfma_f64_16x16x4f64.  The matrix lives in the MFMA
+// a 16-deep GEMM and runs on v_mfma_f64_16x16x4f64.  Three barriers per
+// panel.  The matrix lives in the MFMA
 // accumulator layout: 16 x 16 tiles, wave (wr, wc) of the 2 x 2 wave grid
 // owns row blocks wr, wr+2, ... and column blocks wc, wc+2, ...; lane l,
 // register r of a tile holds element (4*r + l/16, l%16) (operands: A(i, k)
@@ -35,10 +39,8 @@ template <int NB>
 struct BlkShared {
     static constexpr int N16 = 16 * NB;
     double pan[2][16][N16 + 1];   // panel columns, [k][row], double-buffered by panel parity
-    double rb[16][N16];           // old pivot rows of the panel, [s][col]
-    double prow2[2][16];          // pivot rows of the panel factorisation, by step parity
-    double colk[2][N16];          // column kk of the panel before step kk, by step parity
-    unsigned long long wkey[2][4];   // per-wave pivot keys, by step parity
+    double rb[17][N16];           // old pivot rows of the panel, [s][col]; row 16 takes discarded writes
+    double prow[16];              // current pivot row of the panel factorisation
     int ipos[N16];
     int perm[N16];
     int info;
@@ -64,116 +66,101 @@ __device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v,
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Maximum over the 64 lanes of a wave (all lanes active), returned
-// wave-uniform: DPP within rows of 16 (quad swaps, half-row and row
-// mirrors), then the four row maxima through readlane.
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-    unsigned long long u;
-    u = dpp_u64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v = u > v ? u : v;
-    u = dpp_u64<0x4E>(v);    // quad_perm [2,3,0,1]
-    v = u > v ? u : v;
-    u = dpp_u64<0x141>(v);   // row_half_mirror
-    v = u > v ? u : v;
-    u = dpp_u64<0x140>(v);   // row_mirror
-    v = u > v ? u : v;
-    unsigned long long m = readlane_u64(v, 0);
-    u = readlane_u64(v, 16);
-    m = u > m ? u : m;
-    u = readlane_u64(v, 32);
-    m = u > m ? u : m;
-    u = readlane_u64(v, 48);
-    return u > m ? u : m;
+// Maximum over the 64 lanes (all active), wave-uniform: DPP max within rows
+// of 16 (quad swaps, half-row and row mirrors), then the four row maxima
+// through readlane.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false));
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0), b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32), d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a, b), max(c, d));
 }
 
-// Pivot steps p0 .. p0+kend-1 on the panel pan[k][row] with all four waves:
-// thread t holds column c = t & 15 of rows g + 16m (g = t >> 4, m < NB).
-// Per step: the holders of column kk form their pivot keys (|value| bits
-// with the row in the low byte, so ties go to the lowest row) and publish
-// the column; one barrier; every thread reduces the four wave keys, the
-// holders of the pivot row publish it; a second barrier; every thread
-// updates its NB entries.  Same operations, in the same order per entry,
-// as the one-wave panel.
+// Pivot steps p0 .. p0+kend-1 on the panel pan[k][row] (one wave).  The
+// pivot key is the upper word of |value| with the row in its low byte: the
+// largest magnitude to 12 mantissa bits, ties to the lowest row.
 template <int NB>
-__device__ __forceinline__ void blk_panel4(BlkShared<NB> &sh, double (*pan)[16 * NB + 1], int ni, int p0, int kend,
-                                           int tag) {
+__device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * NB + 1], int ni, int p0, int kend,
+                                          int tag) {
     constexpr int N16 = 16 * NB;
-    const int tid = threadIdx.x, c = tid & 15, g = tid >> 4, w = tid >> 6, l = tid & 63;
-    double v[NB];
-    bool av[NB];
-#pragma unroll
-    for (int m = 0; m < NB; m++) {
-        const int r = g + 16 * m;
-        v[m] = pan[c][r];
-        av[m] = (r < ni) && sh.ipos[r] < 0;
-    }
+    constexpr int RPL = (N16 + 63) / 64;
+    const int l = threadIdx.x & 63;
+    double v[RPL][16];
+    bool av[RPL];
     bool zero = false;
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const int r = l + 64 * q;
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[q][k] = (r < N16) ? pan[k][r] : 0.0;
+        av[q] = (r < ni) && sh.ipos[r] < 0;
+    }
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
         if (kk >= kend) continue;   // uniform; keeps the trip count static
-        const int par = kk & 1;
-        unsigned long long key = 0;
-        if (c == kk) {
+        unsigned best = 0;
 #pragma unroll
-            for (int m = 0; m < NB; m++) {
-                const int r = g + 16 * m;
-                const unsigned long long k1 =
-                    ((unsigned long long)__double_as_longlong(fabs(v[m])) & ~0xffull) | (unsigned)(255 - r);
-                if (av[m]) key = k1 > key ? k1 : key;
-                sh.colk[par][r] = v[m];
+        for (int q = 0; q < RPL; q++) {
+            const int r = l + 64 * q;
+            const unsigned key =
+                ((unsigned)(__double_as_longlong(fabs(v[q][kk])) >> 32) & ~0xffu) | (unsigned)(255 - r);
+            best = av[q] && key > best ? key : best;
+        }
+        best = wave_max_u32(best);
+        const int p = best ? 255 - (int)(best & 0xffu) : 0;
+        zero = zero || (best >> 8) == 0u;   // zero (or no) pivot
+        const int pl = p & 63, qp = p >> 6;
+        // the owner lane hands the pivot row to the wave through LDS (in order within the wave)
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (q == qp && l == pl) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) sh.prow[k] = v[q][k];
             }
-        }
-        // the wave's maximum: column kk sits in lanes kk, 16+kk, 32+kk, 48+kk
-        unsigned long long wk = readlane_u64(key, kk), u;
-        u = readlane_u64(key, 16 + kk);
-        wk = u > wk ? u : wk;
-        u = readlane_u64(key, 32 + kk);
-        wk = u > wk ? u : wk;
-        u = readlane_u64(key, 48 + kk);
-        wk = u > wk ? u : wk;
-        if (l == 0) sh.wkey[par][w] = wk;
-        __syncthreads();
-        unsigned long long best = sh.wkey[par][0];
+        // lanes exchange data here: without the barrier the compiler may let the
+        // other lanes read before the owner's (exec-masked) writes
+        __builtin_amdgcn_wave_barrier();
+        double rs[16];
 #pragma unroll
-        for (int q = 1; q < 4; q++) {
-            const unsigned long long b2 = sh.wkey[par][q];
-            best = b2 > best ? b2 : best;
-        }
-        const unsigned blo = (unsigned)(best & 0xffffffffull), bhi = (unsigned)(best >> 32);
-        const int p = (blo | bhi) ? 255 - (int)(blo & 0xffu) : 0;
-        zero = zero || (((blo >> 8) | bhi) == 0u);
-        const int pg = p & 15, pm = p >> 4;
-        double f[NB];
+        for (int k = 0; k < 16; k++) rs[k] = sh.prow[k];
+        const double inv = 1.0 / rs[kk];
 #pragma unroll
-        for (int m = 0; m < NB; m++) f[m] = sh.colk[par][g + 16 * m];
-        if (g == pg) {
-            double pv = v[0];
+        for (int k = 0; k < 16; k++) rs[k] = rs[k] * inv;
 #pragma unroll
-            for (int m = 1; m < NB; m++) pv = (m == pm) ? v[m] : pv;
-            sh.prow2[par][c] = pv;
-        }
-        __syncthreads();
-        const double piv = sh.prow2[par][kk];
-        const double inv = 1.0 / piv;
-        const double rsc = sh.prow2[par][c] * inv;
+        for (int q = 0; q < RPL; q++) {
+            const double f = v[q][kk];
 #pragma unroll
-        for (int m = 0; m < NB; m++) {
-            const double nv = (c == kk) ? -f[m] * inv : fma(-f[m], rsc, v[m]);
-            const bool isp = (g == pg) && (m == pm);
-            v[m] = isp ? ((c == kk) ? inv : rsc) : nv;
-            if (isp) av[m] = false;
+            for (int k = 0; k < 16; k++) v[q][k] = fma(-f, rs[k], v[q][k]);
+            v[q][kk] = -f * inv;
         }
-        if (tid == 0) {
-            sh.ipos[p] = p0 + kk;
-            sh.perm[p0 + kk] = p;
+        // the pivot row itself: the scaled row, inv in the pivot column
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (q == qp && l == pl) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) v[q][k] = rs[k];
+                v[q][kk] = inv;
+                av[q] = false;
+            }
+        if (l == pl) sh.ipos[p] = p0 + kk;
+        if (l == 0) sh.perm[p0 + kk] = p;
+    }
+    if (zero && l == 0 && sh.info == 0) sh.info = tag;
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const int r = l + 64 * q;
+        if (r < N16) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) pan[k][r] = v[q][k];
         }
     }
-    if (zero && tid == 0 && sh.info == 0) sh.info = tag;
-#pragma unroll
-    for (int m = 0; m < NB; m++) pan[c][g + 16 * m] = v[m];
-    for (int s2 = kend; s2 < 16; s2++)
-        for (int cc = tid; cc < N16; cc += NT) sh.rb[s2][cc] = 0.0;
+    for (int s = kend; s < 16; s++)
+        for (int c = l; c < N16; c += 64) sh.rb[s][c] = 0.0;
 }
+
 
 // Inverts the matrix held in acc (MFMA tile layout) and writes the inverse
 // transposed to X (X[m][i] = inv[i][m]).  A zero pivot records `tag` in
@@ -205,7 +192,7 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
             }
         }
         __syncthreads();
-        blk_panel4<NB>(sh, pan, ni, p0, kend, tag);
+        if (w == ((pb + (int)blockIdx.x) & 3)) blk_panel<NB>(sh, pan, ni, p0, kend, tag);
         __syncthreads();
         // panel tiles take the factored panel; the other tiles hand their
         // pivot-row entries to rb and clear them
@@ -224,12 +211,13 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
 #pragma unroll
                     for (int r = 0; r < 4; r++) acc[ti][tj][r] = pan[lc][16 * bi + 4 * r + lg];
                 } else {
+                    // branch-free: entries of other rows go to the spare row 16 of rb
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (sr[r] >= 0 && sr[r] < 16) {
-                            sh.rb[sr[r]][16 * bj + lc] = acc[ti][tj][r];
-                            acc[ti][tj][r] = 0.0;
-                        }
+                    for (int r = 0; r < 4; r++) {
+                        const int s2 = (unsigned)sr[r] < 16u ? sr[r] : 16;
+                        sh.rb[s2][16 * bj + lc] = acc[ti][tj][r];
+                        acc[ti][tj][r] = s2 < 16 ? 0.0 : acc[ti][tj][r];
+                    }
                 }
             }
         }

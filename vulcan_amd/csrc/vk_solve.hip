@@ -550,12 +550,12 @@ extern "C" int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, f
 }
 
 namespace {
-__global__ void k_selftest_wave_max(const unsigned long long *in, unsigned long long *out) {
-    out[threadIdx.x] = wave_max_u64(in[threadIdx.x]);
+__global__ void k_selftest_wave_max(const unsigned *in, unsigned *out) {
+    out[threadIdx.x] = wave_max_u32(in[threadIdx.x]);
 }
 }  // namespace
 
-extern "C" int vk_selftest_wave_max_u64(const unsigned long long *d_in, unsigned long long *d_out, void *stream) {
+extern "C" int vk_selftest_wave_max_u32(const unsigned *d_in, unsigned *d_out, void *stream) {
     if (!d_in || !d_out) return VK_EINVAL;
     hipLaunchKernelGGL(k_selftest_wave_max, dim3(1), dim3(64), 0, (hipStream_t)stream, d_in, d_out);
     VK_CHECK(hipGetLastError());

@@ -1,31 +1,31 @@
-"""Launch the dominant kernels a few times, for counter collection.
+"""Run a few eager Ros2 attempts of the bench workloads, for counter collection.
 
     rocprofv3 --pmc FETCH_SIZE -d <dir> -o pmc -- python3 -m vulcan_amd.profile_kernels
     rocprofv3 --pmc WRITE_SIZE -d <dir> -o pmc -- python3 -m vulcan_amd.profile_kernels
 
-Dispatches, in order: a plain device copy of COPY_MIB MiB (torch's copy
-kernel: COPY_MIB MiB read and written, the calibration of the byte
-counters), then for the one-column workload (nz = 150, ni = 93) the lhs
-assembly and `reps` block factorisations with the tuned kernel, then the
-same for the 256-column batch.  The factor variant used is printed.
+Dispatches, in order: a plain device copy of COPY_MIB MiB (COPY_MIB MiB read
+and written: the calibration of the byte counters), then for the one-column
+workload (nz = 150, ni = 93) and for the 256-column batch (nz uniform on
+[80, 200]) `attempts` eager attempts each (every kernel of an attempt is a
+separate dispatch; no graph replay).  The factor variant picked by the tuner
+is printed, so that the per-dispatch counters can be grouped per attempt.
 """
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 
 import torch
 
 from . import _lib
 from . import workloads as W
-from .device import ColumnBatch, DevicePlan, dptr, stream_handle
+from .device import ColumnBatch, DevicePlan
 
 COPY_MIB = 512
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('--reps', type=int, default=3)
+    ap.add_argument('--attempts', type=int, default=3)
     ap.add_argument('--no-batched', action='store_true')
     a = ap.parse_args(argv)
     lib = _lib.load()
@@ -41,21 +41,12 @@ def main(argv=None):
     for name, cs in cases:
         plan = DevicePlan(cs[0].net)
         b = ColumnBatch(plan, [c.col for c in cs], [c.y0 for c in cs], cs[0].cfg)
-        L, ni = b.layout.L, plan.net.ni
-        D = torch.empty((L, ni, ni), dtype=torch.float64, device='cuda')
-        Di = torch.empty_like(D)
-        Ls = torch.empty((L, ni), dtype=torch.float64, device='cuda')
-        Us = torch.empty_like(Ls)
-        dt = torch.full((len(cs),), 1e-6, dtype=torch.float64, device='cuda')
-        s = stream_handle()
-        _lib.check(lib.vk_lhs(plan.handle, b.layout.handle, C.byref(b.atm), dptr(b.y), dptr(b.k), 1, dptr(dt),
-                              dptr(D), dptr(Ls), dptr(Us), s), 'vk_lhs')
-        for _ in range(a.reps):
-            _lib.check(lib.vk_block_factor(plan.handle, b.layout.handle, dptr(D), dptr(Ls), dptr(Us), dptr(Di), None,
-                                           s), 'vk_block_factor')
+        b.set_graph(False)
         torch.cuda.synchronize()
-        print('%s: L = %d layers, ni = %d, factor variant %d, %d factor launches'
-              % (name, L, ni, lib.vk_plan_factor_variant(plan.handle), a.reps), flush=True)
+        print('%s: L = %d layers, ni = %d, factor variant %d; %d eager attempts follow'
+              % (name, b.layout.L, plan.net.ni, lib.vk_plan_factor_variant(plan.handle), a.attempts), flush=True)
+        b.integrate(a.attempts, use_conv=False)
+        torch.cuda.synchronize()
         b.close()
 
 
