@@ -132,43 +132,61 @@ __global__ void k_copy_d(long n, const double *__restrict__ a, double *__restric
     if (i < n) b[i] = a[i];
 }
 
+// Matrix-vector products of the solves: out[a] = sum_b M[a][b] v[b] with M
+// stored transposed (Mt[b][a]), 256 threads: row a = tid % 128 (+128 ...),
+// half h = tid / 128 of the b range, four independent accumulators per
+// thread so that the loads of a row stay in flight; the two halves are
+// combined by the caller through LDS.
+constexpr int BCR_ST = 256;
+
+__device__ __forceinline__ double tmv_part(const double *__restrict__ Mt, const double *v, int ni, int a, int h) {
+    const int hb = (ni + 1) >> 1, b0 = h * hb, b1 = min(ni, b0 + hb);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = b0;
+    for (; b + 3 < b1; b += 4) {
+        s0 = fma(Mt[(size_t)b * ni + a], v[b], s0);
+        s1 = fma(Mt[(size_t)(b + 1) * ni + a], v[b + 1], s1);
+        s2 = fma(Mt[(size_t)(b + 2) * ni + a], v[b + 2], s2);
+        s3 = fma(Mt[(size_t)(b + 3) * ni + a], v[b + 3], s3);
+    }
+    for (; b < b1; b++) s0 = fma(Mt[(size_t)b * ni + a], v[b], s0);
+    return (s0 + s1) + (s2 + s3);
+}
+
 // solves: forward reduction r_j -= P_j r_l + Q_j r_r at level s
-__global__ void k_bcr_fwd(int ni, int n, const int4 *__restrict__ sv, int slot0, const double *__restrict__ P,
-                          const double *__restrict__ Q, double *__restrict__ r, const vk_colstate *__restrict__ st) {
+__global__ void __launch_bounds__(BCR_ST) k_bcr_fwd(int ni, int n, const int4 *__restrict__ sv, int slot0,
+                                                    const double *__restrict__ P, const double *__restrict__ Q,
+                                                    double *__restrict__ r, const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x;
     if (q >= n) return;
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
     const size_t nn = (size_t)ni * ni;
     const double *Pj = P + (size_t)(slot0 + q) * nn, *Qj = Q + (size_t)(slot0 + q) * nn;
-    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX];
-    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX], pp[2][VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += BCR_ST) {
         rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
         rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
-        double v = r[(size_t)e.x * ni + a];
-        if (e.y >= 0) {
-            double s = 0.0;
-            for (int b = 0; b < ni; b++) s = fma(Pj[(size_t)b * ni + a], rl[b], s);
-            v = v - s;
-        }
-        if (e.z >= 0) {
-            double s = 0.0;
-            for (int b = 0; b < ni; b++) s = fma(Qj[(size_t)b * ni + a], rr[b], s);
-            v = v - s;
-        }
-        r[(size_t)e.x * ni + a] = v;
+    const int h = threadIdx.x >> 7;
+    for (int a = threadIdx.x & 127; a < ni; a += 128) {
+        const double sp = e.y >= 0 ? tmv_part(Pj, rl, ni, a, h) : 0.0;
+        const double sq = e.z >= 0 ? tmv_part(Qj, rr, ni, a, h) : 0.0;
+        pp[h][a] = sp + sq;
     }
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += BCR_ST) r[(size_t)e.x * ni + a] = r[(size_t)e.x * ni + a] - (pp[0][a] + pp[1][a]);
 }
 
 // x_k = A_k^-1 (r_k - B_k x_l - C_k x_r) for the eliminated layers of level s
 // (level 0: diagonal couplings Lsub, Usup); top: x = A^-1 r
-__global__ void k_bcr_bwd(int ni, int n, const int4 *__restrict__ el, const int2 *__restrict__ top, int level,
-                          const double *__restrict__ Dinv, const double *__restrict__ Lsub,
-                          const double *__restrict__ Usup, const double *__restrict__ B, const double *__restrict__ Cc,
-                          const double *__restrict__ r, double *__restrict__ x, const vk_colstate *__restrict__ st) {
+__global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *__restrict__ el,
+                                                    const int2 *__restrict__ top, int level,
+                                                    const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                                                    const double *__restrict__ Usup, const double *__restrict__ B,
+                                                    const double *__restrict__ Cc, const double *__restrict__ r,
+                                                    double *__restrict__ x, const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x;
     if (q >= n) return;
     int k, left = -1, right = -1, c;
@@ -176,38 +194,36 @@ __global__ void k_bcr_bwd(int ni, int n, const int4 *__restrict__ el, const int2
     else { k = top[q].x; c = top[q].y; }
     if (st && st[c].done) return;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX];
-    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX], pp[2][VK_NI_MAX];
+    for (int i = threadIdx.x; i < ni; i += BCR_ST) {
         xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
         xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
+    const int h = threadIdx.x >> 7;
+    if (level > 0) {
+        for (int a = threadIdx.x & 127; a < ni; a += 128) {
+            const double sb = left >= 0 ? tmv_part(B + (size_t)k * nn, xl, ni, a, h) : 0.0;
+            const double sc = right >= 0 ? tmv_part(Cc + (size_t)k * nn, xr, ni, a, h) : 0.0;
+            pp[h][a] = sb + sc;
+        }
+        __syncthreads();
+    }
+    for (int a = threadIdx.x; a < ni; a += BCR_ST) {
         double v = r[(size_t)k * ni + a];
         if (level == 0) {
             if (left >= 0) v = v - Lsub[(size_t)k * ni + a] * xl[a];
             if (right >= 0) v = v - Usup[(size_t)k * ni + a] * xr[a];
         } else {
-            if (left >= 0) {
-                double s = 0.0;
-                for (int b = 0; b < ni; b++) s = fma(B[(size_t)k * nn + (size_t)b * ni + a], xl[b], s);
-                v = v - s;
-            }
-            if (right >= 0) {
-                double s = 0.0;
-                for (int b = 0; b < ni; b++) s = fma(Cc[(size_t)k * nn + (size_t)b * ni + a], xr[b], s);
-                v = v - s;
-            }
+            v = v - (pp[0][a] + pp[1][a]);
         }
         w[a] = v;
     }
     __syncthreads();
-    const double *X = Dinv + (size_t)k * nn;   // X[m][a] = inv[a][m]
-    for (int a = threadIdx.x; a < ni; a += blockDim.x) {
-        double s = 0.0;
-        for (int m = 0; m < ni; m++) s = fma(X[(size_t)m * ni + a], w[m], s);
-        x[(size_t)k * ni + a] = s;
-    }
+    const double *X = Dinv + (size_t)k * nn;   // X[m][a] = inv[a][m]: the transposed inverse
+    for (int a = threadIdx.x & 127; a < ni; a += 128) pp[h][a] = tmv_part(X, w, ni, a, h);
+    __syncthreads();
+    for (int a = threadIdx.x; a < ni; a += BCR_ST) x[(size_t)k * ni + a] = pp[0][a] + pp[1][a];
 }
 
 template <int NB>
@@ -263,7 +279,7 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
     const vk_bcr_sched &b = cols->bcr;
     const long n = (long)cols->L * ni;
     hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
-    const int bt = ni <= 64 ? 64 : (ni <= 128 ? 128 : 192);
+    const int bt = BCR_ST;
     for (int lev = 0; lev < b.nlev; lev++) {
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
         if (ns > 0)

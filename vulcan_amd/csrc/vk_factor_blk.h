@@ -20,20 +20,18 @@ __global__ void __launch_bounds__(NT, MINW) k_factor_blk(int ni, const int *__re
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
     f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
-    if (side == 0) {
-        for (int j = 0; j < m; ++j) {
-            const size_t l = (size_t)l0 + j;
-            blk_load_schur<NB>(acc, ni, D + l * nn, Lsub + l * ni, j > 0 ? Dinv + (l - 1) * nn : nullptr,
-                               j > 0 ? Usup + (l - 1) * ni : nullptr, nullptr, nullptr, nullptr);
-            blk_invert<NB>(acc, ni, Dinv + l * nn, sh, j + 1);
-        }
-    } else {
-        for (int j = nz - 1; j > m; --j) {
-            const size_t l = (size_t)l0 + j;
-            blk_load_schur<NB>(acc, ni, D + l * nn, Usup + l * ni, j < nz - 1 ? Dinv + (l + 1) * nn : nullptr,
-                               j < nz - 1 ? Lsub + (l + 1) * ni : nullptr, nullptr, nullptr, nullptr);
-            blk_invert<NB>(acc, ni, Dinv + l * nn, sh, j + 1);
-        }
+    // one loop for both chains (a single copy of the inversion code, which
+    // is large: the kernel's instruction footprint matters with two
+    // workgroups per CU): side 0 runs j = 0 .. m-1 coupled through Lsub_j /
+    // Usup_{j-1}, side 1 runs j = nz-1 .. m+1 through Usup_j / Lsub_{j+1}
+    const int dir = side == 0 ? 1 : -1, j0 = side == 0 ? 0 : nz - 1, cnt = side == 0 ? m : nz - 1 - m;
+    const double *cown = side == 0 ? Lsub : Usup, *cprev = side == 0 ? Usup : Lsub;
+    for (int t = 0; t < cnt; ++t) {
+        const int j = j0 + dir * t;
+        const size_t l = (size_t)l0 + j, lp = (size_t)((long)l - dir);
+        blk_load_schur<NB>(acc, ni, D + l * nn, cown + l * ni, t > 0 ? Dinv + lp * nn : nullptr,
+                           t > 0 ? cprev + lp * ni : nullptr, nullptr, nullptr, nullptr);
+        blk_invert<NB>(acc, ni, Dinv + l * nn, sh, j + 1);
     }
     if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
 }

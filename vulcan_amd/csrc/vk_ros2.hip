@@ -176,25 +176,42 @@ __global__ void k_atom_col(int natom, const int *__restrict__ off, const double 
 
 // per-column decision.  mode 0: only set delta (plain solver); mode 1: one_step logic.
 // flags: 0 retry, 1 accepted, 2 accepted after giving up, 3 inactive; bit 4: run update_mu_dz
-__global__ void k_decide(int natom, const int *__restrict__ off, const double *__restrict__ part, vk_params prm,
-                         int mode, const double *__restrict__ atom_ini, double *__restrict__ atom_loss,
-                         double *__restrict__ atom_loss_prev, vk_colstate *__restrict__ st,
-                         int *__restrict__ flags, double *__restrict__ thist, int tcap) {
+// One workgroup per column: the column's per-layer partials are staged in
+// LDS with coalesced loads (when they fit), then one thread reduces them in
+// layer order (the order of the reference's sums) and takes the decision.
+constexpr int DECIDE_NT = 256;
+constexpr size_t DECIDE_STAGE_MAX = 48 * 1024;
+
+__global__ void __launch_bounds__(DECIDE_NT) k_decide(int natom, const int *__restrict__ off,
+                                                      const double *__restrict__ part, vk_params prm, int mode,
+                                                      const double *__restrict__ atom_ini,
+                                                      double *__restrict__ atom_loss,
+                                                      double *__restrict__ atom_loss_prev,
+                                                      vk_colstate *__restrict__ st, int *__restrict__ flags,
+                                                      double *__restrict__ thist, int tcap, int stage) {
+    extern __shared__ double dsm[];
     const int c = blockIdx.x;
     vk_colstate s = st[c];
-    if (s.done) { if (flags) flags[c] = 3; return; }
+    if (s.done) { if (flags && threadIdx.x == 0) flags[c] = 3; return; }
     const int l0 = off[c], l1 = off[c + 1];
+    const double *pc = part + (size_t)l0 * NP;
+    if (stage) {
+        for (int e = threadIdx.x; e < (l1 - l0) * NP; e += DECIDE_NT) dsm[e] = pc[e];
+        __syncthreads();
+        pc = dsm;
+    }
+    if (threadIdx.x != 0) return;
     double delta = -1.0;
-    for (int l = l0; l < l1; l++) delta = fmax(delta, part[(size_t)l * NP + VKP_DELTA]);
+    for (int l = l0; l < l1; l++) delta = fmax(delta, pc[(size_t)(l - l0) * NP + VKP_DELTA]);
     if (delta < 0) delta = 0.0;
     s.delta = delta;
     if (mode == 0) { st[c] = s; return; }
     int neg = 0;
     double small = 0.0, nega = 0.0;
     for (int l = l0; l < l1; l++) {
-        neg |= part[(size_t)l * NP + VKP_NEG] != 0.0;
-        small += part[(size_t)l * NP + VKP_SMALL];
-        nega += part[(size_t)l * NP + VKP_NEGY];
+        neg |= pc[(size_t)(l - l0) * NP + VKP_NEG] != 0.0;
+        small += pc[(size_t)(l - l0) * NP + VKP_SMALL];
+        nega += pc[(size_t)(l - l0) * NP + VKP_NEGY];
     }
     s.small_y += fabs(small);
     s.nega_y += fabs(nega);
@@ -202,7 +219,7 @@ __global__ void k_decide(int natom, const int *__restrict__ off, const double *_
     bool lossnan = false;
     for (int a = 0; a < natom; a++) {
         double acc = 0.0;
-        for (int l = l0; l < l1; l++) acc += part[(size_t)l * NP + VKP_ATOM + a];
+        for (int l = l0; l < l1; l++) acc += pc[(size_t)(l - l0) * NP + VKP_ATOM + a];
         const double ini = atom_ini[(size_t)c * natom + a];
         const double loss = (acc - ini) / ini;
         atom_loss[(size_t)c * natom + a] = loss;
@@ -735,8 +752,10 @@ extern "C" int vk_ros2_solver(vk_integrator *it, const vk_atm *atm, const vk_par
     if (r) return r;
     r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 0, false, s);
     if (r) return r;
-    hipLaunchKernelGGL(k_decide, dim3(it->cols->ncols), dim3(1), 0, s, it->plan->natom, it->cols->d_off, it->part,
-                       *prm, 0, nullptr, nullptr, nullptr, d_st, nullptr, nullptr, 0);
+    const size_t dsm = (size_t)it->cols->max_nz * NP * sizeof(double);
+    const int stage = dsm <= DECIDE_STAGE_MAX;
+    hipLaunchKernelGGL(k_decide, dim3(it->cols->ncols), dim3(DECIDE_NT), stage ? dsm : 0, s, it->plan->natom,
+                       it->cols->d_off, it->part, *prm, 0, nullptr, nullptr, nullptr, d_st, nullptr, nullptr, 0, stage);
     VK_CHECK(hipGetLastError());
     return 0;
 }
@@ -749,8 +768,10 @@ static int loop_body(vk_integrator *it, const vk_atm *atm, const vk_geom *g, con
     const vk_columns *cl = it->cols;
     int r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, timed, s);
     if (r) return r;
-    hipLaunchKernelGGL(k_decide, dim3(cl->ncols), dim3(1), 0, s, p->natom, cl->d_off, it->part, *prm, 1,
-                       d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->flags, it->thist, it->tcap);
+    const size_t dsm = (size_t)cl->max_nz * NP * sizeof(double);
+    const int stage = dsm <= DECIDE_STAGE_MAX;
+    hipLaunchKernelGGL(k_decide, dim3(cl->ncols), dim3(DECIDE_NT), stage ? dsm : 0, s, p->natom, cl->d_off, it->part,
+                       *prm, 1, d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->flags, it->thist, it->tcap, stage);
     if (prm->use_update_mu)
         hipLaunchKernelGGL(k_update_mu, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, p->ms, d_ymix, *atm,
                            *g, it->flags, 0, d_st);
