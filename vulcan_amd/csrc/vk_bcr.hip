@@ -99,32 +99,33 @@ __global__ void __launch_bounds__(NT) k_bcr_update(int ni, int n, const int4 *__
     const size_t nn = (size_t)ni * ni, j = e.x;
     double *Ptj = Pt + (size_t)(slot0 + q) * nn, *Qtj = Qt + (size_t)(slot0 + q) * nn;
     __shared__ SlabShared<NB> g;
-    f64x4 acc[(NB + 1) / 2];
-    // P = B_j X_l, Q = C_j X_r: B_j(a, c) = Bt_j[c][a], inv(c, b) = X[b][c]
-    if (e.y >= 0) {
-        slab_zero<NB>(acc);
-        slab_gemm<NB>(acc, ni, r0, Bt + j * nn, 1, ni, Dinv + (size_t)e.y * nn, 1, ni, g);
-        slab_store<NB>(acc, ni, r0, Ptj, true, 1.0, nullptr);
-    }
-    if (e.z >= 0) {
-        slab_zero<NB>(acc);
-        slab_gemm<NB>(acc, ni, r0, Ct + j * nn, 1, ni, Dinv + (size_t)e.z * nn, 1, ni, g);
-        slab_store<NB>(acc, ni, r0, Qtj, true, 1.0, nullptr);
-    }
+    f64x4 acc[(NB + 1) / 2], acc2[(NB + 1) / 2];
+    // P = B_j X_l and Q = C_j X_r together: B_j(a, c) = Bt_j[c][a], inv(c, b) = X[b][c]
+    const bool hl = e.y >= 0, hr = e.z >= 0;
+    slab_zero<NB>(acc);
+    slab_zero<NB>(acc2);
+    if (hl) slab_gemm2<NB, false>(acc, acc2, ni, r0, Bt + j * nn, Dinv + (size_t)e.y * nn,
+                                  hr ? Ct + j * nn : nullptr, hr ? Dinv + (size_t)e.z * nn : nullptr, g);
+    else if (hr) slab_gemm2<NB, false>(acc2, acc, ni, r0, Ct + j * nn, Dinv + (size_t)e.z * nn, nullptr, nullptr, g);
+    if (hl) slab_store<NB>(acc, ni, r0, Ptj, true, 1.0, nullptr);
+    if (hr) slab_store<NB>(acc2, ni, r0, Qtj, true, 1.0, nullptr);
     __threadfence_block();
     __syncthreads();
-    // A_j -= P C_l + Q B_r
+    // A_j -= P C_l + Q B_r (one accumulator)
     slab_zero<NB>(acc);
-    if (e.y >= 0) slab_gemm<NB>(acc, ni, r0, Ptj, 1, ni, Ct + (size_t)e.y * nn, 1, ni, g);
-    if (e.z >= 0) slab_gemm<NB>(acc, ni, r0, Qtj, 1, ni, Bt + (size_t)e.z * nn, 1, ni, g);
+    if (hl) slab_gemm2<NB, true>(acc, acc2, ni, r0, Ptj, Ct + (size_t)e.y * nn, hr ? Qtj : nullptr,
+                                 hr ? Bt + (size_t)e.z * nn : nullptr, g);
+    else if (hr) slab_gemm2<NB, true>(acc, acc2, ni, r0, Qtj, Bt + (size_t)e.z * nn, nullptr, nullptr, g);
     slab_store<NB>(acc, ni, r0, A + j * nn, false, -1.0, A + j * nn);
-    // new couplings: B_j = -P B_l (to l's left neighbour), C_j = -Q C_r
+    // new couplings together: B_j = -P B_l (to l's left neighbour), C_j = -Q C_r
+    const bool nl = fl & 1, nr = fl & 2;
     slab_zero<NB>(acc);
-    if (fl & 1) slab_gemm<NB>(acc, ni, r0, Ptj, 1, ni, Bt + (size_t)e.y * nn, 1, ni, g);
+    slab_zero<NB>(acc2);
+    if (nl) slab_gemm2<NB, false>(acc, acc2, ni, r0, Ptj, Bt + (size_t)e.y * nn, nr ? Qtj : nullptr,
+                                  nr ? Ct + (size_t)e.z * nn : nullptr, g);
+    else if (nr) slab_gemm2<NB, false>(acc2, acc, ni, r0, Qtj, Ct + (size_t)e.z * nn, nullptr, nullptr, g);
     slab_store<NB>(acc, ni, r0, Bt + j * nn, true, -1.0, nullptr);
-    slab_zero<NB>(acc);
-    if (fl & 2) slab_gemm<NB>(acc, ni, r0, Qtj, 1, ni, Ct + (size_t)e.z * nn, 1, ni, g);
-    slab_store<NB>(acc, ni, r0, Ct + j * nn, true, -1.0, nullptr);
+    slab_store<NB>(acc2, ni, r0, Ct + j * nn, true, -1.0, nullptr);
 }
 
 __global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
@@ -226,6 +227,114 @@ __global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *_
     for (int a = threadIdx.x; a < ni; a += BCR_ST) x[(size_t)k * ni + a] = pp[0][a] + pp[1][a];
 }
 
+// Solve sweeps for ni <= 128: 1024 threads, thread t owns row a = t % 128
+// and part h = t / 128 of the b range (chunks of ceil(ni/8) <= 16); every
+// matrix entry a thread needs is loaded into registers before the first
+// product, so each kernel waits for one round of loads, and the eight
+// partial sums of a row are combined through LDS in part order.
+constexpr int MV_NT = 1024, MV_RP = 128, MV_SP = 8, MV_CH = 16;
+
+__device__ __forceinline__ void mv_load(double (&m)[MV_CH], const double *__restrict__ Mt, int ni, int a, int h,
+                                        int ch) {
+#pragma unroll
+    for (int i = 0; i < MV_CH; i++) {
+        const int b = h * ch + i;
+        m[i] = (i < ch && b < ni && a < ni) ? Mt[(size_t)b * ni + a] : 0.0;
+    }
+}
+
+__device__ __forceinline__ double mv_dot(const double (&m)[MV_CH], const double *v, int ni, int h, int ch) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < MV_CH; i += 2) {
+        const int b = h * ch + i;
+        if (i < ch && b < ni) s0 = fma(m[i], v[b], s0);
+        if (i + 1 < ch && b + 1 < ni) s1 = fma(m[i + 1], v[b + 1], s1);
+    }
+    return s0 + s1;
+}
+
+__global__ void __launch_bounds__(MV_NT) k_bcr_fwd_s(int ni, int n, const int4 *__restrict__ sv, int slot0,
+                                                     const double *__restrict__ P, const double *__restrict__ Q,
+                                                     double *__restrict__ r, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int4 e = sv[q];
+    if (st && st[e.w].done) return;
+    const size_t nn = (size_t)ni * ni;
+    const int a = threadIdx.x % MV_RP, h = threadIdx.x / MV_RP, ch = (ni + MV_SP - 1) / MV_SP;
+    double mp[MV_CH], mq[MV_CH];
+    mv_load(mp, P + (size_t)(slot0 + q) * nn, e.y >= 0 ? ni : 0, a, h, ch);
+    mv_load(mq, Q + (size_t)(slot0 + q) * nn, e.z >= 0 ? ni : 0, a, h, ch);
+    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX], pp[MV_SP][MV_RP];
+    for (int i = threadIdx.x; i < ni; i += MV_NT) {
+        rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
+        rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
+    }
+    __syncthreads();
+    pp[h][a] = mv_dot(mp, rl, ni, h, ch) + mv_dot(mq, rr, ni, h, ch);
+    __syncthreads();
+    if (threadIdx.x < ni) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < MV_SP; k++) t += pp[k][threadIdx.x];
+        r[(size_t)e.x * ni + threadIdx.x] = r[(size_t)e.x * ni + threadIdx.x] - t;
+    }
+}
+
+__global__ void __launch_bounds__(MV_NT) k_bcr_bwd_s(int ni, int n, const int4 *__restrict__ el,
+                                                     const int2 *__restrict__ top, int level,
+                                                     const double *__restrict__ Dinv, const double *__restrict__ Lsub,
+                                                     const double *__restrict__ Usup, const double *__restrict__ B,
+                                                     const double *__restrict__ Cc, const double *__restrict__ r,
+                                                     double *__restrict__ x, const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    int k, left = -1, right = -1, c;
+    if (el) { const int4 e = el[q]; k = e.x; left = e.y; right = e.z; c = e.w; }
+    else { k = top[q].x; c = top[q].y; }
+    if (st && st[c].done) return;
+    const size_t nn = (size_t)ni * ni;
+    const int a = threadIdx.x % MV_RP, h = threadIdx.x / MV_RP, ch = (ni + MV_SP - 1) / MV_SP;
+    double mb[MV_CH], mc[MV_CH], mx[MV_CH];
+    mv_load(mb, B + (size_t)k * nn, (level > 0 && left >= 0) ? ni : 0, a, h, ch);
+    mv_load(mc, Cc + (size_t)k * nn, (level > 0 && right >= 0) ? ni : 0, a, h, ch);
+    mv_load(mx, Dinv + (size_t)k * nn, ni, a, h, ch);   // X[m][a] = inv[a][m]: the transposed inverse
+    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX], pp[MV_SP][MV_RP];
+    for (int i = threadIdx.x; i < ni; i += MV_NT) {
+        xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
+        xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
+    }
+    __syncthreads();
+    if (level > 0) {
+        pp[h][a] = mv_dot(mb, xl, ni, h, ch) + mv_dot(mc, xr, ni, h, ch);
+        __syncthreads();
+    }
+    if (threadIdx.x < ni) {
+        const int i = threadIdx.x;
+        double v = r[(size_t)k * ni + i];
+        if (level == 0) {
+            if (left >= 0) v = v - Lsub[(size_t)k * ni + i] * xl[i];
+            if (right >= 0) v = v - Usup[(size_t)k * ni + i] * xr[i];
+        } else {
+            double t = 0.0;
+#pragma unroll
+            for (int p2 = 0; p2 < MV_SP; p2++) t += pp[p2][i];
+            v = v - t;
+        }
+        w[i] = v;
+    }
+    __syncthreads();
+    pp[h][a] = mv_dot(mx, w, ni, h, ch);
+    __syncthreads();
+    if (threadIdx.x < ni) {
+        double t = 0.0;
+#pragma unroll
+        for (int p2 = 0; p2 < MV_SP; p2++) t += pp[p2][threadIdx.x];
+        x[(size_t)k * ni + threadIdx.x] = t;
+    }
+}
+
 template <int NB>
 int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const double *Lsub, const double *Usup,
                         double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
@@ -279,19 +388,22 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
     const vk_bcr_sched &b = cols->bcr;
     const long n = (long)cols->L * ni;
     hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
-    const int bt = BCR_ST;
+    // ni <= 128: the register-prefetching 1024-thread sweeps
+    const bool small = ni <= MV_RP;
+    auto fwd = small ? k_bcr_fwd_s : k_bcr_fwd;
+    auto bwd = small ? k_bcr_bwd_s : k_bcr_bwd;
+    const int bt = small ? MV_NT : BCR_ST;
     for (int lev = 0; lev < b.nlev; lev++) {
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
-        if (ns > 0)
-            hipLaunchKernelGGL(k_bcr_fwd, dim3(ns), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
+        if (ns > 0) hipLaunchKernelGGL(fwd, dim3(ns), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
     }
-    hipLaunchKernelGGL(k_bcr_bwd, dim3(b.ntop), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, 0, Dinv,
-                       Lsub, Usup, ws->B, ws->C, tmp, out, st);
+    hipLaunchKernelGGL(bwd, dim3(b.ntop), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, 0, Dinv, Lsub,
+                       Usup, ws->B, ws->C, tmp, out, st);
     for (int lev = b.nlev - 1; lev >= 0; lev--) {
         const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
         if (ne > 0)
-            hipLaunchKernelGGL(k_bcr_bwd, dim3(ne), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, lev,
-                               Dinv, Lsub, Usup, ws->B, ws->C, tmp, out, st);
+            hipLaunchKernelGGL(bwd, dim3(ne), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, lev, Dinv,
+                               Lsub, Usup, ws->B, ws->C, tmp, out, st);
     }
     VK_CHECK(hipGetLastError());
     return 0;

@@ -302,98 +302,107 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
 }
 
 // Products for the cyclic-reduction updates, one 32-row output slab per
-// workgroup: acc += op(A)[r0:r0+32, :] op(B) over k < ni with
-//     A(a, c) = Ab[a*sar + c*sac],  B(c, b) = Bb[c*sbr + b*sbc].
-// Output tiles are 16 x 16 (MFMA C layout: lane l, register r holds row
-// 4r + l/16, column l%16 of the tile); wave w owns row block w & 1 of the
-// slab and column blocks (w >> 1) + 2t.  K runs in chunks of SLAB_KC staged
-// through LDS; the next chunk's global loads are issued into registers
-// before the current chunk's MFMAs, and each chunk is read with the
-// operand's unit-stride index fastest across threads (coalesced whichever
-// way the operand is stored).
-constexpr int SLAB_KC = 32;
+// workgroup, two products at a time:
+//     acc1 += A1[r0:r0+32, :] B1,   acc2 (or acc1 if SAME) += A2[r0:r0+32, :] B2
+// over k < ni, with every operand stored transposed:
+//     A(a, c) = At[c*ni + a],  B(c, b) = Bt[b*ni + c].
+// A null A2 skips the second product.  Output tiles are 16 x 16 (MFMA C
+// layout: lane l, register r holds row 4r + l/16, column l%16 of the tile);
+// wave w owns row block w & 1 of the slab and column blocks (w >> 1) + 2t.
+// K runs in chunks of SLAB_KC staged through LDS; the next chunk's global
+// loads are issued into registers before the current chunk's MFMAs, and
+// each chunk is read with the operand's unit-stride index fastest across
+// threads (A: the row a, B: the inner index c).
+constexpr int SLAB_KC = 16;
 
 template <int NB>
 struct SlabShared {
-    double sa[SLAB_KC][33];
-    double sb[SLAB_KC][16 * NB + 1];
+    double sa[2][SLAB_KC][33];
+    double sb[2][SLAB_KC][16 * NB + 1];
 };
 
 template <int NB>
 struct SlabRegs {
     static constexpr int PA = SLAB_KC * 32 / NT, PB = SLAB_KC * 16 * NB / NT;
-    double a[PA], b[PB];
+    double a[2][PA], b[2][PB];
 };
 
-// element e of a chunk of KC x n: (k, r), the unit-stride index fastest
-__device__ __forceinline__ void chunk_kr(int e, int n, bool rfast, int &k, int &r) {
-    const int k1 = e / n, r1 = e - k1 * n;
-    const int r2 = e / SLAB_KC, k2 = e - r2 * SLAB_KC;
-    k = rfast ? k1 : k2;
-    r = rfast ? r1 : r2;
-}
-
 template <int NB>
-__device__ __forceinline__ void slab_load(SlabRegs<NB> &R, int ni, int r0, int k0, const double *__restrict__ Ab,
-                                          long sar, long sac, const double *__restrict__ Bb, long sbr, long sbc) {
-    constexpr int N16 = 16 * NB;
-    const bool afast = sar == 1, bfast = sbc == 1;
+__device__ __forceinline__ void slab_load(SlabRegs<NB> &R, int u, int ni, int r0, int k0,
+                                          const double *__restrict__ At, const double *__restrict__ Bt) {
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PA; q++) {
-        int k, r;
-        chunk_kr(threadIdx.x + NT * q, 32, afast, k, r);
+        const int e = threadIdx.x + NT * q, k = e >> 5, r = e & 31;   // a = r0 + r fastest
         const int a = r0 + r, c = k0 + k;
-        R.a[q] = (a < ni && c < ni) ? Ab[(long)a * sar + (long)c * sac] : 0.0;
+        R.a[u][q] = (a < ni && c < ni) ? At[(size_t)c * ni + a] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PB; q++) {
-        int k, r;
-        chunk_kr(threadIdx.x + NT * q, N16, bfast, k, r);
+        const int e = threadIdx.x + NT * q, r = e / SLAB_KC, k = e - r * SLAB_KC;   // c = k0 + k fastest
         const int c = k0 + k;
-        R.b[q] = (r < ni && c < ni) ? Bb[(long)c * sbr + (long)r * sbc] : 0.0;
+        R.b[u][q] = (r < ni && c < ni) ? Bt[(size_t)r * ni + c] : 0.0;
     }
 }
 
 template <int NB>
-__device__ __forceinline__ void slab_stage(const SlabRegs<NB> &R, SlabShared<NB> &g, long sar, long sbc) {
-    const bool afast = sar == 1, bfast = sbc == 1;
+__device__ __forceinline__ void slab_stage(const SlabRegs<NB> &R, int u, SlabShared<NB> &g) {
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PA; q++) {
-        int k, r;
-        chunk_kr(threadIdx.x + NT * q, 32, afast, k, r);
-        g.sa[k][r] = R.a[q];
+        const int e = threadIdx.x + NT * q;
+        g.sa[u][e >> 5][e & 31] = R.a[u][q];
     }
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PB; q++) {
-        int k, r;
-        chunk_kr(threadIdx.x + NT * q, 16 * NB, bfast, k, r);
-        g.sb[k][r] = R.b[q];
+        const int e = threadIdx.x + NT * q, r = e / SLAB_KC;
+        g.sb[u][e - r * SLAB_KC][r] = R.b[u][q];
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void slab_gemm(f64x4 (&acc)[(NB + 1) / 2], int ni, int r0, const double *__restrict__ Ab,
-                                          long sar, long sac, const double *__restrict__ Bb, long sbr, long sbc,
-                                          SlabShared<NB> &g) {
+template <int NB, bool SAME>
+__device__ __forceinline__ void slab_gemm2(f64x4 (&acc1)[(NB + 1) / 2], f64x4 (&acc2)[(NB + 1) / 2], int ni, int r0,
+                                           const double *__restrict__ A1, const double *__restrict__ B1,
+                                           const double *__restrict__ A2, const double *__restrict__ B2,
+                                           SlabShared<NB> &g) {
     constexpr int TB = (NB + 1) / 2;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int bi = w & 1, wc = w >> 1, lc = l & 15, lk = l >> 4;
     const int nch = (ni + SLAB_KC - 1) / SLAB_KC;
+    const bool two = A2 != nullptr;
     SlabRegs<NB> R;
-    slab_load<NB>(R, ni, r0, 0, Ab, sar, sac, Bb, sbr, sbc);
+    slab_load<NB>(R, 0, ni, r0, 0, A1, B1);
+    if (two) slab_load<NB>(R, 1, ni, r0, 0, A2, B2);
     for (int ch = 0; ch < nch; ch++) {
         __syncthreads();   // the previous chunk's reads of g are done
-        slab_stage<NB>(R, g, sar, sbc);
+        slab_stage<NB>(R, 0, g);
+        if (two) slab_stage<NB>(R, 1, g);
         __syncthreads();
-        if (ch + 1 < nch) slab_load<NB>(R, ni, r0, (ch + 1) * SLAB_KC, Ab, sar, sac, Bb, sbr, sbc);
+        if (ch + 1 < nch) {
+            slab_load<NB>(R, 0, ni, r0, (ch + 1) * SLAB_KC, A1, B1);
+            if (two) slab_load<NB>(R, 1, ni, r0, (ch + 1) * SLAB_KC, A2, B2);
+        }
 #pragma unroll
         for (int k4 = 0; k4 < SLAB_KC / 4; k4++) {
-            const double a = g.sa[4 * k4 + lk][16 * bi + lc];
+            const double a1 = g.sa[0][4 * k4 + lk][16 * bi + lc];
 #pragma unroll
             for (int tj = 0; tj < TB; tj++) {
                 const int bj = wc + 2 * tj;
                 if (bj >= NB) continue;
-                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g.sb[4 * k4 + lk][16 * bj + lc], acc[tj], 0, 0, 0);
+                acc1[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, g.sb[0][4 * k4 + lk][16 * bj + lc], acc1[tj], 0, 0,
+                                                                 0);
+            }
+            if (two) {
+                const double a2 = g.sa[1][4 * k4 + lk][16 * bi + lc];
+#pragma unroll
+                for (int tj = 0; tj < TB; tj++) {
+                    const int bj = wc + 2 * tj;
+                    if (bj >= NB) continue;
+                    if (SAME)
+                        acc1[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, g.sb[1][4 * k4 + lk][16 * bj + lc],
+                                                                         acc1[tj], 0, 0, 0);
+                    else
+                        acc2[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, g.sb[1][4 * k4 + lk][16 * bj + lc],
+                                                                         acc2[tj], 0, 0, 0);
+                }
             }
         }
     }

@@ -300,9 +300,16 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         if (en.z > en.y) {
             vk_jac_term t = jac_term[en.y];
             J = t.coef * dv[t.slot];
-            for (int u = en.y + 1; u < en.z; u++) {
-                t = jac_term[u];
-                J = J + t.coef * dv[t.slot];
+            // terms in order; four loads in flight per round
+            for (int u = en.y + 1; u < en.z; u += 4) {
+                const vk_jac_term t0 = jac_term[u];
+                const vk_jac_term t1 = u + 1 < en.z ? jac_term[u + 1] : t0;
+                const vk_jac_term t2 = u + 2 < en.z ? jac_term[u + 2] : t0;
+                const vk_jac_term t3 = u + 3 < en.z ? jac_term[u + 3] : t0;
+                J = J + t0.coef * dv[t0.slot];
+                if (u + 1 < en.z) J = J + t1.coef * dv[t1.slot];
+                if (u + 2 < en.z) J = J + t2.coef * dv[t2.slot];
+                if (u + 3 < en.z) J = J + t3.coef * dv[t3.slot];
             }
         }
         if (mode == 0) {

@@ -30,7 +30,7 @@ struct vk_plan {
     int *slot_pair, *slot_sp;
     int *jac_map, *jac_ptr, *jac_slot;
     int njent;                 // entries with chemical terms plus the diagonal
-    int4 *jac_ent;             // (dense index, first term, end of terms, row), by dense index
+    int4 *jac_ent;             // (dense index, first term, end of terms, row), longest term lists first
     vk_jac_term *jac_term;     // jac_coef / jac_slot interleaved
     double *jac_coef;
     std::vector<void *> allocs;

@@ -4,6 +4,7 @@
 // (op.py:2447-2534), update_mu_dz (944-984), renormalisation (909-914),
 // save_step (1087-1105), step_size (3105-3125) and stop/conv (1018-1085).
 #include <cstring>
+#include <algorithm>
 #include <type_traits>
 #include "vk_internal.h"
 #include "vk_device.h"
@@ -202,12 +203,14 @@ __global__ void __launch_bounds__(DECIDE_NT) k_decide(int natom, const int *__re
     }
     if (threadIdx.x != 0) return;
     double delta = -1.0;
+#pragma unroll 8
     for (int l = l0; l < l1; l++) delta = fmax(delta, pc[(size_t)(l - l0) * NP + VKP_DELTA]);
     if (delta < 0) delta = 0.0;
     s.delta = delta;
     if (mode == 0) { st[c] = s; return; }
     int neg = 0;
     double small = 0.0, nega = 0.0;
+#pragma unroll 8
     for (int l = l0; l < l1; l++) {
         neg |= pc[(size_t)(l - l0) * NP + VKP_NEG] != 0.0;
         small += pc[(size_t)(l - l0) * NP + VKP_SMALL];
@@ -219,6 +222,7 @@ __global__ void __launch_bounds__(DECIDE_NT) k_decide(int natom, const int *__re
     bool lossnan = false;
     for (int a = 0; a < natom; a++) {
         double acc = 0.0;
+#pragma unroll 8
         for (int l = l0; l < l1; l++) acc += pc[(size_t)(l - l0) * NP + VKP_ATOM + a];
         const double ini = atom_ini[(size_t)c * natom + a];
         const double loss = (acc - ini) / ini;
@@ -462,6 +466,11 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
             if (jm >= 0) ent.push_back(make_int4(e, n->jac_ptr[jm], n->jac_ptr[jm + 1], row));
             else if (e % n->ni == row) ent.push_back(make_int4(e, 0, 0, row));
         }
+        // heaviest term lists first: the lanes of a wave then carry term
+        // lists of similar length (the diagonal entries, with the longest
+        // lists, share a few waves instead of lengthening every wave)
+        std::stable_sort(ent.begin(), ent.end(),
+                         [](const int4 &x, const int4 &y) { return (x.z - x.y) > (y.z - y.y); });
         p->njent = (int)ent.size();
         r = upload(v, &p->jac_ent, ent.data(), ent.size());
     }
