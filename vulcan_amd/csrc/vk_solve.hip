@@ -90,8 +90,9 @@ __device__ __forceinline__ void rotate(double (&b)[DEP][CH]) {
     }
 }
 
-// y_i = sum_m Xt[m][i] v[m] from the registers of the eight row parts; the
-// result is valid in threads h == 0 (i < ni).  Two barriers.
+// y_i = sum_m Xt[m][i] v[m] from the registers of the row parts; the result
+// is valid in threads h == 0 (i < ni).  One barrier (part is double-buffered
+// by the callers).
 template <int SP, int RP, int CH>
 __device__ __forceinline__ double solve_matvec(const double (&cur)[CH], const double *v, double (*part)[RP], int i,
                                                int h, int m0) {
@@ -117,9 +118,11 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
     const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double v[SP * CH];
-    __shared__ double part[SP][RP];
-    for (int q = tid; q < SP * CH; q += NTS) v[q] = 0.0;
+    // v and part double-buffered by step parity: two barriers per step; the
+    // right-hand side and coupling of the next step are loaded a step ahead
+    __shared__ double v[2][SP * CH];
+    __shared__ double part[2][SP][RP];
+    for (int q = tid; q < 2 * SP * CH; q += NTS) v[q / (SP * CH)][q % (SP * CH)] = 0.0;
     const int j0 = side == 0 ? 0 : nz - 1, dj = side == 0 ? 1 : -1;
     const int nsteps = side == 0 ? m : nz - 1 - m;
     const double *cpl = side == 0 ? Lsub : Usup;
@@ -128,24 +131,34 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
 #pragma unroll
     for (int d = 0; d < DEP; d++)
         if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)(l0 + j0 + d * dj) * nn, ni, i, m0);
-    double zi = 0.0;
+    const bool row = h == 0 && i < ni;
+    double zi = 0.0, in_n = 0.0, cp_n = 0.0;
+    if (row && nsteps > 0) {
+        in_n = in[(size_t)(l0 + j0) * ni + i];
+        cp_n = cpl[(size_t)(l0 + j0) * ni + i];
+    }
     __syncthreads();
     for (int s = 0; s < nsteps; s++) {
         const size_t l = (size_t)(l0 + j0 + dj * s);
-        if (h == 0 && i < ni) {
-            double w = in[l * ni + i];
-            if (s > 0) w = w - cpl[l * ni + i] * zi;
-            v[i] = w;
+        const int pb = s & 1;
+        const double in_s = in_n, cp_s = cp_n;
+        if (row && s + 1 < nsteps) {
+            in_n = in[(l + dj) * ni + i];
+            cp_n = cpl[(l + dj) * ni + i];
+        }
+        if (row) {
+            double w = in_s;
+            if (s > 0) w = w - cp_s * zi;
+            v[pb][i] = w;
         }
         __syncthreads();
-        const double z = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
-        if (h == 0 && i < ni) {
+        const double z = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, m0);
+        if (row) {
             tmp[l * ni + i] = z;
             zi = z;
         }
         rotate<DEP, CH>(buf);
         if (s + DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)(l0 + j0 + dj * (s + DEP)) * nn, ni, i, m0);
-        __syncthreads();
     }
     // park b_m in the unused slot m so the backward kernel never reads `in`
     // (which may alias `out`)
@@ -166,9 +179,9 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
     const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double v[SP * CH];
-    __shared__ double part[SP][RP];
-    for (int q = tid; q < SP * CH; q += NTS) v[q] = 0.0;
+    __shared__ double v[2][SP * CH];
+    __shared__ double part[2][SP][RP];
+    for (int q = tid; q < 2 * SP * CH; q += NTS) v[q / (SP * CH)][q % (SP * CH)] = 0.0;
     const size_t lm = (size_t)l0 + m;
     const int dj = side == 0 ? -1 : 1;
     const int nsteps = 1 + (side == 0 ? m : nz - 1 - m);     // the middle layer, then the chain outwards
@@ -178,32 +191,43 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
 #pragma unroll
     for (int d = 0; d < DEP; d++)
         if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)((long)lm + (long)d * dj) * nn, ni, i, m0);
+    const bool row = h == 0 && i < ni;
     // x_m = M^-1 (b_m - Lsub_m z_{m-1} - Usup_m w_{m+1}); both sides compute it
-    if (h == 0 && i < ni) {
+    if (row) {
         double w = tmp[lm * ni + i];
         if (m > 0) w = w - Lsub[lm * ni + i] * tmp[(lm - 1) * ni + i];
         if (m < nz - 1) w = w - Usup[lm * ni + i] * tmp[(lm + 1) * ni + i];
-        v[i] = w;
+        v[0][i] = w;
+    }
+    // coupling and forward-sweep value of the next layer, a step ahead
+    double cp_n = 0.0, t_n = 0.0;
+    if (row && nsteps > 1) {
+        cp_n = cpl[(lm + dj) * ni + i];
+        t_n = tmp[(lm + dj) * ni + i];
     }
     __syncthreads();
-    double xi = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
-    if (side == 0 && h == 0 && i < ni) out[lm * ni + i] = xi;
+    double xi = solve_matvec<SP, RP, CH>(buf[0], v[0], part[0], i, h, m0);
+    if (side == 0 && row) out[lm * ni + i] = xi;
     rotate<DEP, CH>(buf);
     if (DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)DEP * dj) * nn, ni, i, m0);
-    __syncthreads();
     for (int s = 1; s < nsteps; s++) {
         const size_t l = lm + (long)dj * s;
-        if (h == 0 && i < ni) v[i] = cpl[l * ni + i] * xi;
+        const int pb = s & 1;
+        const double cp_s = cp_n, t_s = t_n;
+        if (row && s + 1 < nsteps) {
+            cp_n = cpl[(l + dj) * ni + i];
+            t_n = tmp[(l + dj) * ni + i];
+        }
+        if (row) v[pb][i] = cp_s * xi;
         __syncthreads();
-        const double y = solve_matvec<SP, RP, CH>(buf[0], v, part, i, h, m0);
-        if (h == 0 && i < ni) {
-            xi = tmp[l * ni + i] - y;
+        const double y = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, m0);
+        if (row) {
+            xi = t_s - y;
             out[l * ni + i] = xi;
         }
         rotate<DEP, CH>(buf);
         if (s + DEP < nsteps)
             load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)dj * (s + DEP)) * nn, ni, i, m0);
-        __syncthreads();
     }
 }
 
