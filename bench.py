@@ -10,10 +10,13 @@ photolysis channels), all FP64, started from dt = 1e-10 s.  Inputs are
 resident on the device before timing starts.
 
 The JSON line also carries
-* ``roofline`` for the dominant kernel (block-Thomas factorisation), with the
-  average launch time measured by HIP events on the launch stream (eager
-  attempts right after the timed region; the timed region itself replays
-  the captured per-attempt HIP graph);
+* ``roofline`` for the dominant kernel, the block factorisation of the Ros2
+  lhs (for one column the tuner picks block cyclic reduction: a sequence of
+  launches, timed as one), with the average duration measured by HIP events
+  on the launch stream (eager attempts right after the timed region; the
+  timed region itself replays the captured per-attempt HIP graph) and the
+  HBM traffic per factorisation measured with rocprofv3 byte counters
+  (FACTOR_TRAFFIC below);
 * ``cpu_baseline``: the CPU oracle (reference algorithm: dense lhs, band
   repack, two banded LAPACK solves per attempt) timed on the host in a
   single-threaded subprocess on a bounded sample;
@@ -37,6 +40,12 @@ VARIANTS = ['register Gauss-Jordan chain', 'blocked Gauss-Jordan chain (MFMA Sch
             'blocked Gauss-Jordan chain, two workgroups per CU', 'two-wave Gauss-Jordan chain',
             'block cyclic reduction (MFMA Schur updates)']
 FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
+# HBM bytes per factorisation of the nz = 150, ni = 93 column, by factor
+# variant: FETCH_SIZE + WRITE_SIZE summed over the factorisation kernels of
+# one eager attempt (python3 -m vulcan_amd.profile_kernels under
+# rocprofv3 --pmc, one counter per run), each counter scaled by its reading
+# for a 512 MiB device copy.  Algorithmic: 2 ni^2 x 8 B per layer = 20.8 MB.
+FACTOR_TRAFFIC = {4: 235979499}   # block cyclic reduction: 236.0 MB (11.4x; slab products re-read their right operands)
 
 
 def _sync():
@@ -219,7 +228,9 @@ def main():
                                + ' (chosen by the tuner; per-variant ms for factor + 2 solves: %s)' % s['tuner_ms'],
                      'bound': 'mfma',
                      'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP64_PEAK, 'traffic': None,
+                     'frac': achieved / FP64_PEAK, 'traffic': FACTOR_TRAFFIC.get(s['variant']),
+                     'traffic_unit': 'bytes per factorisation (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)',
+                     'algorithmic_bytes_per_factorisation': nz * 2 * ni * ni * 8,
                      'flops_per_factorisation': flops_launch, 'avg_factorisation_ms': s['factor_ms']},
     }
     if not args.no_batched:
