@@ -39,7 +39,8 @@ template <int NB>
 struct BlkShared {
     static constexpr int N16 = 16 * NB;
     double pan[2][16][N16 + 1];   // panel columns, [k][row], double-buffered by panel parity
-    double rb[17][N16];           // old pivot rows of the panel, [s][col]; row 16 takes discarded writes
+    double rb[16][N16];           // old pivot rows of the panel, [s][col]
+    double dump[NT];              // per-thread target of discarded writes
     double prow[16];              // current pivot row of the panel factorisation
     int ipos[N16];
     int perm[N16];
@@ -211,12 +212,13 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
 #pragma unroll
                     for (int r = 0; r < 4; r++) acc[ti][tj][r] = pan[lc][16 * bi + 4 * r + lg];
                 } else {
-                    // branch-free: entries of other rows go to the spare row 16 of rb
+                    // branch-free: entries of other rows go to the thread's own dump slot
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int s2 = (unsigned)sr[r] < 16u ? sr[r] : 16;
-                        sh.rb[s2][16 * bj + lc] = acc[ti][tj][r];
-                        acc[ti][tj][r] = s2 < 16 ? 0.0 : acc[ti][tj][r];
+                        const bool piv = (unsigned)sr[r] < 16u;
+                        double *dst = piv ? &sh.rb[sr[r]][16 * bj + lc] : &sh.dump[tid];
+                        *dst = acc[ti][tj][r];
+                        acc[ti][tj][r] = piv ? 0.0 : acc[ti][tj][r];
                     }
                 }
             }
