@@ -311,21 +311,27 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
 // A null A2 skips the second product.  Output tiles are 16 x 16 (MFMA C
 // layout: lane l, register r holds row 4r + l/16, column l%16 of the tile);
 // wave w owns row block w & 1 of the slab and column blocks (w >> 1) + 2t.
-// K runs in chunks of SLAB_KC staged through LDS; the next chunk's global
+// K runs in chunks of SlabKC<NB> staged through LDS; the next chunk's global
 // loads are issued into registers before the current chunk's MFMAs, and
 // each chunk is read with the operand's unit-stride index fastest across
 // threads (A: the row a, B: the inner index c).
-constexpr int SLAB_KC = 16;
+// chunk depth: as deep as two staged operand pairs fit in 64 KiB of LDS
+template <int NB>
+struct SlabKC {
+    static constexpr int value = NB <= 7 ? 24 : 16;
+};
 
 template <int NB>
 struct SlabShared {
-    double sa[2][SLAB_KC][33];
-    double sb[2][SLAB_KC][16 * NB + 1];
+    static constexpr int KC = SlabKC<NB>::value;
+    double sa[2][KC][33];
+    double sb[2][KC][16 * NB + 1];
 };
 
 template <int NB>
 struct SlabRegs {
-    static constexpr int PA = SLAB_KC * 32 / NT, PB = SLAB_KC * 16 * NB / NT;
+    static constexpr int KC = SlabKC<NB>::value;
+    static constexpr int PA = KC * 32 / NT, PB = (KC * 16 * NB + NT - 1) / NT;
     double a[2][PA], b[2][PB];
 };
 
@@ -338,11 +344,12 @@ __device__ __forceinline__ void slab_load(SlabRegs<NB> &R, int u, int ni, int r0
         const int a = r0 + r, c = k0 + k;
         R.a[u][q] = (a < ni && c < ni) ? At[(size_t)c * ni + a] : 0.0;
     }
+    constexpr int KC = SlabKC<NB>::value;
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PB; q++) {
-        const int e = threadIdx.x + NT * q, r = e / SLAB_KC, k = e - r * SLAB_KC;   // c = k0 + k fastest
+        const int e = threadIdx.x + NT * q, r = e / KC, k = e - r * KC;   // c = k0 + k fastest
         const int c = k0 + k;
-        R.b[u][q] = (r < ni && c < ni) ? Bt[(size_t)r * ni + c] : 0.0;
+        R.b[u][q] = (e < KC * 16 * NB && r < ni && c < ni) ? Bt[(size_t)r * ni + c] : 0.0;
     }
 }
 
@@ -353,10 +360,11 @@ __device__ __forceinline__ void slab_stage(const SlabRegs<NB> &R, int u, SlabSha
         const int e = threadIdx.x + NT * q;
         g.sa[u][e >> 5][e & 31] = R.a[u][q];
     }
+    constexpr int KC = SlabKC<NB>::value;
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB>::PB; q++) {
-        const int e = threadIdx.x + NT * q, r = e / SLAB_KC;
-        g.sb[u][e - r * SLAB_KC][r] = R.b[u][q];
+        const int e = threadIdx.x + NT * q, r = e / KC;
+        if (e < KC * 16 * NB) g.sb[u][e - r * KC][r] = R.b[u][q];
     }
 }
 
@@ -368,7 +376,8 @@ __device__ __forceinline__ void slab_gemm2(f64x4 (&acc1)[(NB + 1) / 2], f64x4 (&
     constexpr int TB = (NB + 1) / 2;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int bi = w & 1, wc = w >> 1, lc = l & 15, lk = l >> 4;
-    const int nch = (ni + SLAB_KC - 1) / SLAB_KC;
+    constexpr int KC = SlabKC<NB>::value;
+    const int nch = (ni + KC - 1) / KC;
     const bool two = A2 != nullptr;
     SlabRegs<NB> R;
     slab_load<NB>(R, 0, ni, r0, 0, A1, B1);
@@ -379,11 +388,11 @@ __device__ __forceinline__ void slab_gemm2(f64x4 (&acc1)[(NB + 1) / 2], f64x4 (&
         if (two) slab_stage<NB>(R, 1, g);
         __syncthreads();
         if (ch + 1 < nch) {
-            slab_load<NB>(R, 0, ni, r0, (ch + 1) * SLAB_KC, A1, B1);
-            if (two) slab_load<NB>(R, 1, ni, r0, (ch + 1) * SLAB_KC, A2, B2);
+            slab_load<NB>(R, 0, ni, r0, (ch + 1) * KC, A1, B1);
+            if (two) slab_load<NB>(R, 1, ni, r0, (ch + 1) * KC, A2, B2);
         }
 #pragma unroll
-        for (int k4 = 0; k4 < SLAB_KC / 4; k4++) {
+        for (int k4 = 0; k4 < KC / 4; k4++) {
             const double a1 = g.sa[0][4 * k4 + lk][16 * bi + lc];
 #pragma unroll
             for (int tj = 0; tj < TB; tj++) {
