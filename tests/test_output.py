@@ -1,0 +1,93 @@
+"""`.vul` output (op.py:3216-3256): key sets of store.py and the access
+patterns of VULCAN's plot_py readers (plot_vulcan.py:64-86,
+plot_evolution.py:39-59)."""
+import pickle
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from vulcan_amd import output as OUT
+from vulcan_amd import workloads as W
+
+gpu = pytest.mark.gpu
+
+# Variables.var_save (store.py:86-88) and var_evol_save (store.py:92) of the reference
+REF_VAR_SAVE = ['k', 'y', 'ymix', 'y_ini', 't', 'dt', 'longdy', 'longdydt', 'atom_ini', 'atom_sum', 'atom_loss',
+                'atom_conden', 'aflux_change', 'Rf']
+REF_VAR_EVOL_SAVE = ['y_time', 't_time']
+
+
+def _host_objects(case, nsteps=5):
+    col, net = case.col, case.net
+    nz, ni = case.y0.shape
+    ymix = case.y0 / case.y0.sum(axis=1, keepdims=True)
+    var = SimpleNamespace(k={j + 1: np.full(nz, float(j)) for j in range(net.nr)}, y=case.y0, ymix=ymix,
+                          y_ini=case.y0.copy(), t=1.0, dt=0.1, longdy=0.5, longdydt=0.2,
+                          atom_ini={a: 1.0 for a in net.atoms()}, atom_sum={a: 1.0 for a in net.atoms()},
+                          atom_loss={a: 0.0 for a in net.atoms()}, atom_conden={a: 0.0 for a in net.atoms()},
+                          aflux_change=0.0, Rf={2 * p + 1: net.equation(p) for p in range(net.npairs)},
+                          y_time=[case.y0 * (1 + s) for s in range(nsteps)], t_time=list(np.arange(nsteps) * 1.0))
+    atm = SimpleNamespace(pco=col.pco, pico=col.pico, zco=col.zco, zmco=0.5 * (col.zco[:-1] + col.zco[1:]),
+                          n_0=col.M, Tco=col.Tco, Kzz=col.Kzz)
+    para = SimpleNamespace(count=nsteps, end_case=1, solver_str='solver')
+    return var, atm, para
+
+
+def test_vul_file_has_the_reference_keys_and_reads_like_vulcan_output(tmp_path):
+    case = W.small_case(nz=20)
+    var, atm, para = _host_objects(case)
+    assert OUT.VAR_SAVE == REF_VAR_SAVE and OUT.VAR_EVOL_SAVE == REF_VAR_EVOL_SAVE
+    path = OUT.save_vul(str(tmp_path / 'out' / 'run.vul'), var, atm, para, case.net.species, case.net.nr,
+                        save_evolution=True, save_evo_frq=2)
+    with open(path, 'rb') as f:
+        data = pickle.load(f)            # a file this test wrote itself
+    assert set(data) == {'variable', 'atm', 'parameter'}
+    v = data['variable']
+    assert set(v) == set(REF_VAR_SAVE) | set(REF_VAR_EVOL_SAVE) | {'species', 'nr'}
+    species = v['species']
+    sp = species[3]
+    # plot_vulcan.py:74, 77
+    assert np.array_equal(v['ymix'][:, species.index(sp)], var.ymix[:, 3])
+    assert np.allclose(v['y_ini'][:, species.index(sp)] / data['atm']['n_0'], case.y0[:, 3] / case.col.M)
+    assert np.array_equal(data['atm']['pco'] / 1e6, case.col.pco / 1e6)
+    assert data['atm']['zco'][1:].shape == (20,) and data['atm']['zmco'].shape == (20,)
+    # plot_evolution.py:57: every save_evo_frq-th state (op.py:3244-3246)
+    yt = np.array(v['y_time'])
+    assert yt.shape == (3, 20, case.net.ni) and np.array_equal(v['t_time'], [0.0, 2.0, 4.0])
+    assert np.array_equal(yt[:, 5, species.index(sp)] / float(data['atm']['n_0'][5]),
+                          np.array([case.y0[5, 3] * (1 + s) for s in (0, 2, 4)]) / case.col.M[5])
+    assert v['nr'] == case.net.nr and len(v['k']) == case.net.nr and v['Rf'][1] == case.net.equation(0)
+    assert data['parameter']['end_case'] == 1
+
+
+def test_human_readable_output_is_the_dictionary_text(tmp_path):
+    case = W.small_case(nz=8)
+    var, atm, para = _host_objects(case, nsteps=2)
+    path = OUT.save_vul(str(tmp_path / 'run.txt'), var, atm, para, case.net.species, case.net.nr,
+                        human_readable=True)
+    text = open(path).read()
+    assert text.startswith("{'variable': {'species': [") and "'parameter'" in text
+
+
+@gpu
+def test_device_column_written_as_vul(tmp_path):
+    import torch
+    from vulcan_amd.device import ColumnBatch, DevicePlan
+    assert torch.cuda.is_available()
+    cases = [W.small_case(nz=20), W.small_case(nz=24)]
+    plan = DevicePlan(cases[0].net)
+    b = ColumnBatch(plan, [c.col for c in cases], [c.y0 for c in cases], cases[0].cfg, tune=False)
+    b.reset_element_baseline()
+    b.integrate(4, use_conv=False)
+    torch.cuda.synchronize()
+    var, atm, para = OUT.column_result(b, 1, cases[1].col, cases[1].y0)
+    path = OUT.save_vul(str(tmp_path / 'c1.vul'), var, atm, para, cases[1].net.species, cases[1].net.nr)
+    with open(path, 'rb') as f:
+        data = pickle.load(f)
+    v = data['variable']
+    assert np.array_equal(v['y'], b.column('y', 1)) and v['y'].shape == (24, cases[1].net.ni)
+    assert np.array_equal(v['k'][1], b.column('k', 1)[:, 0])
+    assert data['parameter']['count'] == int(b.host_state()['count'][1]) > 0
+    assert data['atm']['zco'].shape == (25,) and np.all(np.diff(data['atm']['zco']) > 0)
+    b.close()
