@@ -88,6 +88,10 @@ def test_device_column_written_as_vul(tmp_path):
     v = data['variable']
     assert np.array_equal(v['y'], b.column('y', 1)) and v['y'].shape == (24, cases[1].net.ni)
     assert np.array_equal(v['k'][1], b.column('k', 1)[:, 0])
-    assert data['parameter']['count'] == int(b.host_state()['count'][1]) > 0
+    st = b.host_state()[1]
+    assert data['parameter']['count'] == int(st['count']) and data['variable']['t'] == float(st['t'])
+    # every attempt is either accepted or counted as a reject
+    assert data['parameter']['count'] + data['parameter']['delta_count'] + data['parameter']['nega_count'] \
+        + data['parameter']['loss_count'] == int(st['attempts']) == 4
     assert data['atm']['zco'].shape == (25,) and np.all(np.diff(data['atm']['zco']) > 0)
     b.close()
