@@ -43,14 +43,32 @@ __global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 
 }
 
 // level 0: couplings are the diagonals Lsub (B) and Usup (C); elementwise,
-// over the transposed index tt = b*ni + a (coalesced P, Q, B, C and inverses)
+// over the transposed index tt = b*ni + a (coalesced P, Q, B, C and
+// inverses), each block's index range split over the grid's y dimension
+// Workgroups n .. n+ne-1 form the back-substitution operators of the
+// eliminated layers instead: XB = A_k^-1 diag(Lsub_k), XC = A_k^-1 diag(Usup_k).
 __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const int *__restrict__ sflag,
                               int slot0, const double *__restrict__ Lsub, const double *__restrict__ Usup,
                               const double *__restrict__ Dinv, double *__restrict__ A, double *__restrict__ Bt,
                               double *__restrict__ Ct, double *__restrict__ Pt, double *__restrict__ Qt,
-                              const vk_colstate *__restrict__ st) {
+                              const int4 *__restrict__ el, int ne, double *__restrict__ XBt,
+                              double *__restrict__ XCt, const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x;
-    if (q >= n) return;
+    if (q >= n + ne) return;
+    if (q >= n) {
+        const int4 e = el[q - n];
+        if (st && st[e.w].done) return;
+        const size_t nn = (size_t)ni * ni, k = e.x;
+        const double *X = Dinv + k * nn;     // X[b][a] = inv[a][b]
+        const int span = (ni * ni + gridDim.y - 1) / gridDim.y, t0 = blockIdx.y * span;
+        const int t1 = min(ni * ni, t0 + span);
+        for (int tt = t0 + threadIdx.x; tt < t1; tt += blockDim.x) {
+            const int b = tt / ni;
+            XBt[k * nn + tt] = X[tt] * Lsub[k * ni + b];
+            XCt[k * nn + tt] = X[tt] * Usup[k * ni + b];
+        }
+        return;
+    }
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
     const int fl = sflag[q];
@@ -58,7 +76,9 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
     const size_t slot = (size_t)(slot0 + q) * nn;
     const double *XL = e.y >= 0 ? Dinv + (size_t)e.y * nn : nullptr;   // X[m][i] = inv[i][m]
     const double *XR = e.z >= 0 ? Dinv + (size_t)e.z * nn : nullptr;
-    for (int tt = threadIdx.x; tt < ni * ni; tt += blockDim.x) {
+    const int span = (ni * ni + gridDim.y - 1) / gridDim.y, t0 = blockIdx.y * span;
+    const int t1 = min(ni * ni, t0 + span);
+    for (int tt = t0 + threadIdx.x; tt < t1; tt += blockDim.x) {
         const int b = tt / ni, a = tt - b * ni;
         double upd = 0.0;
         if (XL) {
@@ -84,21 +104,38 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
 // levels >= 1: dense couplings, products on MFMA; one workgroup per
 // (surviving layer, 32-row slab).  A slab's output rows depend only on the
 // same rows of the left operands, so the slabs are independent.
+// Workgroups n .. n+ne-1 (in x) form the back-substitution operators of the
+// level's eliminated layers, XB = A_k^-1 B_k and XC = A_k^-1 C_k (their
+// couplings are final: only survivors' couplings change at this level).
 template <int NB>
 __global__ void __launch_bounds__(NT) k_bcr_update(int ni, int n, const int4 *__restrict__ sv,
                                                    const int *__restrict__ sflag, int slot0,
                                                    const double *__restrict__ Dinv, double *__restrict__ A,
                                                    double *__restrict__ Bt, double *__restrict__ Ct,
                                                    double *__restrict__ Pt, double *__restrict__ Qt,
-                                                   const vk_colstate *__restrict__ st) {
+                                                   const int4 *__restrict__ el, int ne, double *__restrict__ XBt,
+                                                   double *__restrict__ XCt, const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x, r0 = 32 * blockIdx.y;
-    if (q >= n || r0 >= ni) return;
+    if (q >= n + ne || r0 >= ni) return;
+    __shared__ SlabShared<NB> g;
+    if (q >= n) {
+        const int4 e = el[q - n];
+        if (st && st[e.w].done) return;
+        const size_t nn = (size_t)ni * ni, k = e.x;
+        f64x4 xb[(NB + 1) / 2], xc[(NB + 1) / 2];
+        slab_zero<NB>(xb);
+        slab_zero<NB>(xc);
+        // A_k^-1 (a, c) = X[c][a]: the transposed-storage inverse is the At operand
+        slab_gemm2<NB, false>(xb, xc, ni, r0, Dinv + k * nn, Bt + k * nn, Dinv + k * nn, Ct + k * nn, g);
+        slab_store<NB>(xb, ni, r0, XBt + k * nn, true, 1.0, nullptr);
+        slab_store<NB>(xc, ni, r0, XCt + k * nn, true, 1.0, nullptr);
+        return;
+    }
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
     const int fl = sflag[q];
     const size_t nn = (size_t)ni * ni, j = e.x;
     double *Ptj = Pt + (size_t)(slot0 + q) * nn, *Qtj = Qt + (size_t)(slot0 + q) * nn;
-    __shared__ SlabShared<NB> g;
     f64x4 acc[(NB + 1) / 2], acc2[(NB + 1) / 2];
     // P = B_j X_l and Q = C_j X_r together: B_j(a, c) = Bt_j[c][a], inv(c, b) = X[b][c]
     const bool hl = e.y >= 0, hr = e.z >= 0;
@@ -180,14 +217,13 @@ __global__ void __launch_bounds__(BCR_ST) k_bcr_fwd(int ni, int n, const int4 *_
     for (int a = threadIdx.x; a < ni; a += BCR_ST) r[(size_t)e.x * ni + a] = r[(size_t)e.x * ni + a] - (pp[0][a] + pp[1][a]);
 }
 
-// x_k = A_k^-1 (r_k - B_k x_l - C_k x_r) for the eliminated layers of level s
-// (level 0: diagonal couplings Lsub, Usup); top: x = A^-1 r
+// x_k = A_k^-1 r_k - XB_k x_l - XC_k x_r for the eliminated layers of a level
+// (XB = A_k^-1 B_k, XC = A_k^-1 C_k, formed at factorisation); top: x = A^-1 r
 __global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *__restrict__ el,
-                                                    const int2 *__restrict__ top, int level,
-                                                    const double *__restrict__ Dinv, const double *__restrict__ Lsub,
-                                                    const double *__restrict__ Usup, const double *__restrict__ B,
-                                                    const double *__restrict__ Cc, const double *__restrict__ r,
-                                                    double *__restrict__ x, const vk_colstate *__restrict__ st) {
+                                                    const int2 *__restrict__ top, const double *__restrict__ Dinv,
+                                                    const double *__restrict__ XB, const double *__restrict__ XC,
+                                                    const double *__restrict__ r, double *__restrict__ x,
+                                                    const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x;
     if (q >= n) return;
     int k, left = -1, right = -1, c;
@@ -195,44 +231,32 @@ __global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *_
     else { k = top[q].x; c = top[q].y; }
     if (st && st[c].done) return;
     const size_t nn = (size_t)ni * ni;
-    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX], pp[2][VK_NI_MAX];
+    __shared__ double rk[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX], pp[2][VK_NI_MAX];
     for (int i = threadIdx.x; i < ni; i += BCR_ST) {
+        rk[i] = r[(size_t)k * ni + i];
         xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
         xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
     }
     __syncthreads();
     const int h = threadIdx.x >> 7;
-    if (level > 0) {
-        for (int a = threadIdx.x & 127; a < ni; a += 128) {
-            const double sb = left >= 0 ? tmv_part(B + (size_t)k * nn, xl, ni, a, h) : 0.0;
-            const double sc = right >= 0 ? tmv_part(Cc + (size_t)k * nn, xr, ni, a, h) : 0.0;
-            pp[h][a] = sb + sc;
-        }
-        __syncthreads();
+    for (int a = threadIdx.x & 127; a < ni; a += 128) {
+        double v = tmv_part(Dinv + (size_t)k * nn, rk, ni, a, h);
+        if (left >= 0) v = v - tmv_part(XB + (size_t)k * nn, xl, ni, a, h);
+        if (right >= 0) v = v - tmv_part(XC + (size_t)k * nn, xr, ni, a, h);
+        pp[h][a] = v;
     }
-    for (int a = threadIdx.x; a < ni; a += BCR_ST) {
-        double v = r[(size_t)k * ni + a];
-        if (level == 0) {
-            if (left >= 0) v = v - Lsub[(size_t)k * ni + a] * xl[a];
-            if (right >= 0) v = v - Usup[(size_t)k * ni + a] * xr[a];
-        } else {
-            v = v - (pp[0][a] + pp[1][a]);
-        }
-        w[a] = v;
-    }
-    __syncthreads();
-    const double *X = Dinv + (size_t)k * nn;   // X[m][a] = inv[a][m]: the transposed inverse
-    for (int a = threadIdx.x & 127; a < ni; a += 128) pp[h][a] = tmv_part(X, w, ni, a, h);
     __syncthreads();
     for (int a = threadIdx.x; a < ni; a += BCR_ST) x[(size_t)k * ni + a] = pp[0][a] + pp[1][a];
 }
 
-// Solve sweeps for ni <= 128: 1024 threads, thread t owns row a = t % 128
-// and part h = t / 128 of the b range (chunks of ceil(ni/8) <= 16); every
-// matrix entry a thread needs is loaded into registers before the first
-// product, so each kernel waits for one round of loads, and the eight
-// partial sums of a row are combined through LDS in part order.
-constexpr int MV_NT = 1024, MV_RP = 128, MV_SP = 8, MV_CH = 16;
+// Solve sweeps for ni <= 128: each block's products are split over row
+// groups of 32 (one workgroup each, grid y), so a block's operators are
+// read by several CUs at once; 256 threads, thread t owns row a0 + t % 32
+// and part h = t / 32 of the b range (chunks of ceil(ni/8) <= 16).  Every
+// operator entry a thread needs is loaded into registers before the first
+// product, and the eight partial sums of a row are combined through LDS in
+// part order.
+constexpr int MV_NT = 256, MV_RG = 32, MV_SP = 8, MV_CH = 16, MV_NIMAX = 128;
 
 __device__ __forceinline__ void mv_load(double (&m)[MV_CH], const double *__restrict__ Mt, int ni, int a, int h,
                                         int ch) {
@@ -257,81 +281,63 @@ __device__ __forceinline__ double mv_dot(const double (&m)[MV_CH], const double 
 __global__ void __launch_bounds__(MV_NT) k_bcr_fwd_s(int ni, int n, const int4 *__restrict__ sv, int slot0,
                                                      const double *__restrict__ P, const double *__restrict__ Q,
                                                      double *__restrict__ r, const vk_colstate *__restrict__ st) {
-    const int q = blockIdx.x;
-    if (q >= n) return;
+    const int q = blockIdx.x, a0 = MV_RG * blockIdx.y;
+    if (q >= n || a0 >= ni) return;
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
     const size_t nn = (size_t)ni * ni;
-    const int a = threadIdx.x % MV_RP, h = threadIdx.x / MV_RP, ch = (ni + MV_SP - 1) / MV_SP;
+    const int lr = threadIdx.x % MV_RG, a = a0 + lr, h = threadIdx.x / MV_RG, ch = (ni + MV_SP - 1) / MV_SP;
     double mp[MV_CH], mq[MV_CH];
     mv_load(mp, P + (size_t)(slot0 + q) * nn, e.y >= 0 ? ni : 0, a, h, ch);
     mv_load(mq, Q + (size_t)(slot0 + q) * nn, e.z >= 0 ? ni : 0, a, h, ch);
-    __shared__ double rl[VK_NI_MAX], rr[VK_NI_MAX], pp[MV_SP][MV_RP];
+    __shared__ double rl[MV_NIMAX], rr[MV_NIMAX], pp[MV_SP][MV_RG];
     for (int i = threadIdx.x; i < ni; i += MV_NT) {
         rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
         rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
     }
     __syncthreads();
-    pp[h][a] = mv_dot(mp, rl, ni, h, ch) + mv_dot(mq, rr, ni, h, ch);
+    pp[h][lr] = mv_dot(mp, rl, ni, h, ch) + mv_dot(mq, rr, ni, h, ch);
     __syncthreads();
-    if (threadIdx.x < ni) {
+    if (threadIdx.x < MV_RG && a < ni) {
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < MV_SP; k++) t += pp[k][threadIdx.x];
-        r[(size_t)e.x * ni + threadIdx.x] = r[(size_t)e.x * ni + threadIdx.x] - t;
+        r[(size_t)e.x * ni + a] = r[(size_t)e.x * ni + a] - t;
     }
 }
 
 __global__ void __launch_bounds__(MV_NT) k_bcr_bwd_s(int ni, int n, const int4 *__restrict__ el,
-                                                     const int2 *__restrict__ top, int level,
-                                                     const double *__restrict__ Dinv, const double *__restrict__ Lsub,
-                                                     const double *__restrict__ Usup, const double *__restrict__ B,
-                                                     const double *__restrict__ Cc, const double *__restrict__ r,
-                                                     double *__restrict__ x, const vk_colstate *__restrict__ st) {
-    const int q = blockIdx.x;
-    if (q >= n) return;
+                                                     const int2 *__restrict__ top, const double *__restrict__ Dinv,
+                                                     const double *__restrict__ XB, const double *__restrict__ XC,
+                                                     const double *__restrict__ r, double *__restrict__ x,
+                                                     const vk_colstate *__restrict__ st) {
+    const int q = blockIdx.x, a0 = MV_RG * blockIdx.y;
+    if (q >= n || a0 >= ni) return;
     int k, left = -1, right = -1, c;
     if (el) { const int4 e = el[q]; k = e.x; left = e.y; right = e.z; c = e.w; }
     else { k = top[q].x; c = top[q].y; }
     if (st && st[c].done) return;
     const size_t nn = (size_t)ni * ni;
-    const int a = threadIdx.x % MV_RP, h = threadIdx.x / MV_RP, ch = (ni + MV_SP - 1) / MV_SP;
-    double mb[MV_CH], mc[MV_CH], mx[MV_CH];
-    mv_load(mb, B + (size_t)k * nn, (level > 0 && left >= 0) ? ni : 0, a, h, ch);
-    mv_load(mc, Cc + (size_t)k * nn, (level > 0 && right >= 0) ? ni : 0, a, h, ch);
+    const int lr = threadIdx.x % MV_RG, a = a0 + lr, h = threadIdx.x / MV_RG, ch = (ni + MV_SP - 1) / MV_SP;
+    // x_k = A_k^-1 r_k - XB x_l - XC x_r: one pass over three operators
+    double mx[MV_CH], mb[MV_CH], mc[MV_CH];
     mv_load(mx, Dinv + (size_t)k * nn, ni, a, h, ch);   // X[m][a] = inv[a][m]: the transposed inverse
-    __shared__ double w[VK_NI_MAX], xl[VK_NI_MAX], xr[VK_NI_MAX], pp[MV_SP][MV_RP];
+    mv_load(mb, XB + (size_t)k * nn, left >= 0 ? ni : 0, a, h, ch);
+    mv_load(mc, XC + (size_t)k * nn, right >= 0 ? ni : 0, a, h, ch);
+    __shared__ double rk[MV_NIMAX], xl[MV_NIMAX], xr[MV_NIMAX], pp[MV_SP][MV_RG];
     for (int i = threadIdx.x; i < ni; i += MV_NT) {
+        rk[i] = r[(size_t)k * ni + i];
         xl[i] = left >= 0 ? x[(size_t)left * ni + i] : 0.0;
         xr[i] = right >= 0 ? x[(size_t)right * ni + i] : 0.0;
     }
     __syncthreads();
-    if (level > 0) {
-        pp[h][a] = mv_dot(mb, xl, ni, h, ch) + mv_dot(mc, xr, ni, h, ch);
-        __syncthreads();
-    }
-    if (threadIdx.x < ni) {
-        const int i = threadIdx.x;
-        double v = r[(size_t)k * ni + i];
-        if (level == 0) {
-            if (left >= 0) v = v - Lsub[(size_t)k * ni + i] * xl[i];
-            if (right >= 0) v = v - Usup[(size_t)k * ni + i] * xr[i];
-        } else {
-            double t = 0.0;
-#pragma unroll
-            for (int p2 = 0; p2 < MV_SP; p2++) t += pp[p2][i];
-            v = v - t;
-        }
-        w[i] = v;
-    }
+    pp[h][lr] = (mv_dot(mx, rk, ni, h, ch) - mv_dot(mb, xl, ni, h, ch)) - mv_dot(mc, xr, ni, h, ch);
     __syncthreads();
-    pp[h][a] = mv_dot(mx, w, ni, h, ch);
-    __syncthreads();
-    if (threadIdx.x < ni) {
+    if (threadIdx.x < MV_RG && a < ni) {
         double t = 0.0;
 #pragma unroll
         for (int p2 = 0; p2 < MV_SP; p2++) t += pp[p2][threadIdx.x];
-        x[(size_t)k * ni + threadIdx.x] = t;
+        x[(size_t)k * ni + a] = t;
     }
 }
 
@@ -349,11 +355,13 @@ int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const d
                                (const int2 *)nullptr, cols->d_off, ws->A, Dinv, info, st);
         if (ns > 0) {
             if (lev == 0)
-                hipLaunchKernelGGL(k_bcr_update0, dim3(ns), dim3(NT), 0, s, ni, ns, b.d_surv + s0, b.d_sflag + s0, s0,
-                                   Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+                hipLaunchKernelGGL(k_bcr_update0, dim3(ns + ne, 8), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
+                                   b.d_sflag + s0, s0, Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q,
+                                   b.d_elim + e0, ne, ws->XB, ws->XC, st);
             else
-                hipLaunchKernelGGL((k_bcr_update<NB>), dim3(ns, (ni + 31) / 32), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
-                                   b.d_sflag + s0, s0, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q, st);
+                hipLaunchKernelGGL((k_bcr_update<NB>), dim3(ns + ne, (ni + 31) / 32), dim3(NT), 0, s, ni, ns,
+                                   b.d_surv + s0, b.d_sflag + s0, s0, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q,
+                                   b.d_elim + e0, ne, ws->XB, ws->XC, st);
         }
     }
     hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(b.ntop), dim3(NT), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top,
@@ -388,22 +396,23 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
     const vk_bcr_sched &b = cols->bcr;
     const long n = (long)cols->L * ni;
     hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
-    // ni <= 128: the register-prefetching 1024-thread sweeps
-    const bool small = ni <= MV_RP;
+    // ni <= 128: the row-group sweeps (several workgroups per block)
+    const bool small = ni <= MV_NIMAX;
     auto fwd = small ? k_bcr_fwd_s : k_bcr_fwd;
     auto bwd = small ? k_bcr_bwd_s : k_bcr_bwd;
-    const int bt = small ? MV_NT : BCR_ST;
+    const int bt = small ? MV_NT : BCR_ST, gy = small ? (ni + MV_RG - 1) / MV_RG : 1;
     for (int lev = 0; lev < b.nlev; lev++) {
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
-        if (ns > 0) hipLaunchKernelGGL(fwd, dim3(ns), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
+        if (ns > 0)
+            hipLaunchKernelGGL(fwd, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
     }
-    hipLaunchKernelGGL(bwd, dim3(b.ntop), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, 0, Dinv, Lsub,
-                       Usup, ws->B, ws->C, tmp, out, st);
+    hipLaunchKernelGGL(bwd, dim3(b.ntop, gy), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, Dinv,
+                       ws->XB, ws->XC, tmp, out, st);
     for (int lev = b.nlev - 1; lev >= 0; lev--) {
         const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
         if (ne > 0)
-            hipLaunchKernelGGL(bwd, dim3(ne), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, lev, Dinv,
-                               Lsub, Usup, ws->B, ws->C, tmp, out, st);
+            hipLaunchKernelGGL(bwd, dim3(ne, gy), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, Dinv,
+                               ws->XB, ws->XC, tmp, out, st);
     }
     VK_CHECK(hipGetLastError());
     return 0;

@@ -47,24 +47,11 @@ struct BlkShared {
     int info;
 };
 
-__device__ __forceinline__ double readlane_f64(double x, int lane) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 template <int CTRL>
 __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffull), CTRL, 0xf, 0xf, false);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xf, 0xf, false);
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
-}
-
-__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int lane) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
-    return ((unsigned long long)hi << 32) | lo;
 }
 
 // Maximum over the 64 lanes (all active), wave-uniform: DPP max within rows

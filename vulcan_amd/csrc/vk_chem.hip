@@ -242,10 +242,47 @@ __global__ void __launch_bounds__(BLK) k_rhs(
 // D = c0 I - J - J_diff, Lsub, Usup), one workgroup per layer
 // ---------------------------------------------------------------------------
 
+// one entry of the lhs (mode 1) or of the Jacobian (mode 0): its terms in order
+__device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_ent,
+                                          const vk_jac_term *__restrict__ jac_term, const double *dv, int ni,
+                                          int mode, int mol, double c0, const double *dg0, const double *dg1,
+                                          double *__restrict__ Dl) {
+    const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
+    double J = 0.0;
+    if (en.z > en.y) {
+        vk_jac_term t = jac_term[en.y];
+        J = t.coef * dv[t.slot];
+        // terms in order; four loads in flight per round
+        for (int u = en.y + 1; u < en.z; u += 4) {
+            const vk_jac_term t0 = jac_term[u];
+            const vk_jac_term t1 = u + 1 < en.z ? jac_term[u + 1] : t0;
+            const vk_jac_term t2 = u + 2 < en.z ? jac_term[u + 2] : t0;
+            const vk_jac_term t3 = u + 3 < en.z ? jac_term[u + 3] : t0;
+            J = J + t0.coef * dv[t0.slot];
+            if (u + 1 < en.z) J = J + t1.coef * dv[t1.slot];
+            if (u + 2 < en.z) J = J + t2.coef * dv[t2.slot];
+            if (u + 3 < en.z) J = J + t3.coef * dv[t3.slot];
+        }
+    }
+    if (mode == 0) {
+        Dl[en.x] = J;
+        return;
+    }
+    const int i = en.w;
+    double val = -J;
+    if (en.x == i * ni + i) {
+        val = c0 + val;
+        val = val - dg0[i];
+        if (mol) val = val - dg1[i];
+    }
+    Dl[en.x] = val;
+}
+
 __global__ void __launch_bounds__(BLK) k_jac_lhs(
     int ni, int npairs, int nslot, int nent, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
-    const int *__restrict__ slot_sp, const int4 *__restrict__ jac_ent, const vk_jac_term *__restrict__ jac_term,
+    const int *__restrict__ slot_sp, const int4 *__restrict__ jac_ent, const int *__restrict__ jac_emap, int nheavy,
+    const vk_jac_term *__restrict__ jac_term,
     const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y,
     const double *__restrict__ k, const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
     const double *__restrict__ ysum, const vk_colstate *__restrict__ st, double *__restrict__ D,
@@ -262,9 +299,6 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
     const int nn = ni * ni;
     double *Dl = D + (size_t)l * nn;
-    // entries without a chemical term are zero (the diagonal is rewritten below)
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) Dl[e] = 0.0;
-    __syncthreads();
     const double *kl = k + (size_t)l * 2 * npairs;
     const double m = M[l];
     for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
@@ -293,37 +327,43 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         }
     }
     __syncthreads();
-    // the entries with chemical terms, and every diagonal entry
-    for (int q = threadIdx.x; q < nent; q += blockDim.x) {
-        const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
-        double J = 0.0;
-        if (en.z > en.y) {
-            vk_jac_term t = jac_term[en.y];
-            J = t.coef * dv[t.slot];
-            // terms in order; four loads in flight per round
-            for (int u = en.y + 1; u < en.z; u += 4) {
-                const vk_jac_term t0 = jac_term[u];
-                const vk_jac_term t1 = u + 1 < en.z ? jac_term[u + 1] : t0;
-                const vk_jac_term t2 = u + 2 < en.z ? jac_term[u + 2] : t0;
-                const vk_jac_term t3 = u + 3 < en.z ? jac_term[u + 3] : t0;
-                J = J + t0.coef * dv[t0.slot];
-                if (u + 1 < en.z) J = J + t1.coef * dv[t1.slot];
-                if (u + 2 < en.z) J = J + t2.coef * dv[t2.slot];
-                if (u + 3 < en.z) J = J + t3.coef * dv[t3.slot];
+    // Every dense entry is written once.  Entries with at most JAC_LIGHT
+    // terms (and the structural zeros) in one coalesced pass over the dense
+    // index; the few long term lists (jac_ent[0, nheavy): the list is sorted
+    // by term count) by their own threads alongside.
+    for (int q = threadIdx.x; q < nheavy; q += blockDim.x) jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0,
+                                                                     dg1, Dl);
+#pragma unroll 2
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        // (first term << 4 | term count), -1: structural zero, -2: long term list
+        const int code = jac_emap[e];
+        if (code == -2) continue;
+        double val = 0.0;
+        if (code >= 0) {
+            const int u0 = code >> 4, nt = code & 15;
+            double J = 0.0;
+            if (nt > 0) {
+                vk_jac_term t[VK_JAC_LIGHT];
+#pragma unroll
+                for (int m = 0; m < VK_JAC_LIGHT; m++) t[m] = jac_term[u0 + (m < nt ? m : 0)];
+                J = t[0].coef * dv[t[0].slot];
+#pragma unroll
+                for (int m = 1; m < VK_JAC_LIGHT; m++)
+                    if (m < nt) J = J + t[m].coef * dv[t[m].slot];
+            }
+            if (mode == 0) {
+                val = J;
+            } else {
+                const int i = e / ni;
+                val = -J;
+                if (e == i * ni + i) {
+                    val = c0 + val;
+                    val = val - dg0[i];
+                    if (mol) val = val - dg1[i];
+                }
             }
         }
-        if (mode == 0) {
-            Dl[en.x] = J;
-            continue;
-        }
-        const int i = en.w;
-        double val = -J;
-        if (en.x == i * ni + i) {
-            val = c0 + val;
-            val = val - dg0[i];
-            if (mol) val = val - dg1[i];
-        }
-        Dl[en.x] = val;
+        Dl[e] = val;
     }
 }
 
@@ -388,7 +428,7 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     }
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        plan->njent, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
-                       plan->slot_sp, plan->jac_ent, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
+                       plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
                        dtc, ysum, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;

@@ -6,6 +6,7 @@
 
 #define VK_NI_MAX 160        // largest network the kernels are compiled for
 #define VK_NATOM_MAX 8
+#define VK_JAC_LIGHT 8   // term lists up to this length are evaluated in the dense pass
 
 #define VK_CHECK(expr)                                                   \
     do {                                                                 \
@@ -31,6 +32,8 @@ struct vk_plan {
     int *jac_map, *jac_ptr, *jac_slot;
     int njent;                 // entries with chemical terms plus the diagonal
     int4 *jac_ent;             // (dense index, first term, end of terms, row), longest term lists first
+    int *jac_emap;             // dense index -> (first term << 4 | term count) of a short list, -1 zero, -2 long list
+    int njheavy;               // entries with more than VK_JAC_LIGHT terms (the first ones of jac_ent)
     vk_jac_term *jac_term;     // jac_coef / jac_slot interleaved
     double *jac_coef;
     std::vector<void *> allocs;
@@ -50,9 +53,10 @@ struct vk_bcr_sched {
 };
 
 // workspace of the block cyclic reduction: A, B, C per layer (n x n each),
-// P, Q per surviving (level, layer) entry
+// P, Q per surviving (level, layer) entry, XB = A^-1 B and XC = A^-1 C per
+// eliminated layer (the back-substitution's operators)
 struct vk_bcr_ws {
-    double *A = nullptr, *B = nullptr, *C = nullptr, *P = nullptr, *Q = nullptr;
+    double *A = nullptr, *B = nullptr, *C = nullptr, *P = nullptr, *Q = nullptr, *XB = nullptr, *XC = nullptr;
 };
 
 struct vk_columns {

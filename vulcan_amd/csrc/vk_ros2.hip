@@ -472,7 +472,15 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
         std::stable_sort(ent.begin(), ent.end(),
                          [](const int4 &x, const int4 &y) { return (x.z - x.y) > (y.z - y.y); });
         p->njent = (int)ent.size();
+        p->njheavy = 0;
+        std::vector<int> emap((size_t)n->ni * n->ni, -1);
+        for (int q = 0; q < p->njent; q++) {
+            const int nt = ent[q].z - ent[q].y;
+            if (nt > VK_JAC_LIGHT) p->njheavy = q + 1;
+            emap[ent[q].x] = nt > VK_JAC_LIGHT ? -2 : (ent[q].y << 4) | nt;
+        }
         r = upload(v, &p->jac_ent, ent.data(), ent.size());
+        r = r ? r : upload(v, &p->jac_emap, emap.data(), emap.size());
     }
     if (!r) {
         std::vector<int2> ce((size_t)n->nchem);
@@ -663,6 +671,8 @@ static int ensure_bcr(vk_integrator *it) {
     r = r ? r : dalloc(v, &it->bcr.C, L * nn);
     r = r ? r : dalloc(v, &it->bcr.P, ns * nn);
     r = r ? r : dalloc(v, &it->bcr.Q, ns * nn);
+    r = r ? r : dalloc(v, &it->bcr.XB, L * nn);
+    r = r ? r : dalloc(v, &it->bcr.XC, L * nn);
     return r;
 }
 

@@ -78,7 +78,7 @@ __device__ __forceinline__ void load_chunk(double (&r)[CH], const double *__rest
 // layers in flight per thread: fewer for larger blocks (128 registers per thread at 1024 threads)
 template <int NB>
 struct SolveDepth {
-    static constexpr int value = NB <= 5 ? 3 : (NB <= 7 ? 2 : 1);
+    static constexpr int value = NB <= 4 ? 4 : (NB <= 5 ? 3 : (NB <= 7 ? 2 : 1));
 };
 
 template <int DEP, int CH>
@@ -421,12 +421,14 @@ struct BcrTmp {
             hipMallocAsync((void **)&ws.B, sizeof(double) * L * nn, s) != hipSuccess ||
             hipMallocAsync((void **)&ws.C, sizeof(double) * L * nn, s) != hipSuccess ||
             hipMallocAsync((void **)&ws.P, sizeof(double) * ns * nn, s) != hipSuccess ||
-            hipMallocAsync((void **)&ws.Q, sizeof(double) * ns * nn, s) != hipSuccess)
+            hipMallocAsync((void **)&ws.Q, sizeof(double) * ns * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.XB, sizeof(double) * L * nn, s) != hipSuccess ||
+            hipMallocAsync((void **)&ws.XC, sizeof(double) * L * nn, s) != hipSuccess)
             return VK_ENOMEM;
         return 0;
     }
     ~BcrTmp() {
-        for (double *p : {ws.A, ws.B, ws.C, ws.P, ws.Q})
+        for (double *p : {ws.A, ws.B, ws.C, ws.P, ws.Q, ws.XB, ws.XC})
             if (p) (void)hipFreeAsync(p, s);
     }
 };
