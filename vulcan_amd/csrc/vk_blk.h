@@ -298,85 +298,84 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
 // A null A2 skips the second product.  Output tiles are 16 x 16 (MFMA C
 // layout: lane l, register r holds row 4r + l/16, column l%16 of the tile);
 // wave w owns row block w & 1 of the slab and column blocks (w >> 1) + 2t.
-// K runs in chunks of SlabKC<NB> staged through LDS; the next chunk's global
-// loads are issued into registers before the current chunk's MFMAs, and
-// each chunk is read with the operand's unit-stride index fastest across
-// threads (A: the row a, B: the inner index c).
-// chunk depth: as deep as two staged operand pairs fit in 64 KiB of LDS
+// K runs in chunks of KC staged through LDS; the next chunk's global loads
+// are issued into registers before the current chunk's MFMAs, and each
+// chunk is read with the operand's unit-stride index fastest across threads
+// (A: the row a, B: the inner index c).  Each chunk costs about one global
+// load latency, so deep chunks suit launches that fit the CUs one workgroup
+// each, shallow ones keep two or three workgroups per CU resident.
+// Two staged operand pairs take 16 KC (34 + 16 NB) bytes of LDS.
 template <int NB>
 struct SlabKC {
-    static constexpr int value = NB <= 7 ? 24 : 16;
+    static constexpr int shallow = NB <= 7 ? 24 : 16;   // <= 56 KiB
+    static constexpr int deep = NB <= 6 ? 48 : 32;      // <= 98 KiB (gfx950: 160 KiB per workgroup)
 };
 
-template <int NB>
+template <int NB, int KC>
 struct SlabShared {
-    static constexpr int KC = SlabKC<NB>::value;
     double sa[2][KC][33];
     double sb[2][KC][16 * NB + 1];
 };
 
-template <int NB>
+template <int NB, int KC>
 struct SlabRegs {
-    static constexpr int KC = SlabKC<NB>::value;
+    static_assert(KC % 8 == 0, "chunk depth: a multiple of 8");
     static constexpr int PA = KC * 32 / NT, PB = (KC * 16 * NB + NT - 1) / NT;
     double a[2][PA], b[2][PB];
 };
 
-template <int NB>
-__device__ __forceinline__ void slab_load(SlabRegs<NB> &R, int u, int ni, int r0, int k0,
+template <int NB, int KC>
+__device__ __forceinline__ void slab_load(SlabRegs<NB, KC> &R, int u, int ni, int r0, int k0,
                                           const double *__restrict__ At, const double *__restrict__ Bt) {
 #pragma unroll
-    for (int q = 0; q < SlabRegs<NB>::PA; q++) {
+    for (int q = 0; q < SlabRegs<NB, KC>::PA; q++) {
         const int e = threadIdx.x + NT * q, k = e >> 5, r = e & 31;   // a = r0 + r fastest
         const int a = r0 + r, c = k0 + k;
         R.a[u][q] = (a < ni && c < ni) ? At[(size_t)c * ni + a] : 0.0;
     }
-    constexpr int KC = SlabKC<NB>::value;
 #pragma unroll
-    for (int q = 0; q < SlabRegs<NB>::PB; q++) {
+    for (int q = 0; q < SlabRegs<NB, KC>::PB; q++) {
         const int e = threadIdx.x + NT * q, r = e / KC, k = e - r * KC;   // c = k0 + k fastest
         const int c = k0 + k;
         R.b[u][q] = (e < KC * 16 * NB && r < ni && c < ni) ? Bt[(size_t)r * ni + c] : 0.0;
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void slab_stage(const SlabRegs<NB> &R, int u, SlabShared<NB> &g) {
+template <int NB, int KC>
+__device__ __forceinline__ void slab_stage(const SlabRegs<NB, KC> &R, int u, SlabShared<NB, KC> &g) {
 #pragma unroll
-    for (int q = 0; q < SlabRegs<NB>::PA; q++) {
+    for (int q = 0; q < SlabRegs<NB, KC>::PA; q++) {
         const int e = threadIdx.x + NT * q;
         g.sa[u][e >> 5][e & 31] = R.a[u][q];
     }
-    constexpr int KC = SlabKC<NB>::value;
 #pragma unroll
-    for (int q = 0; q < SlabRegs<NB>::PB; q++) {
+    for (int q = 0; q < SlabRegs<NB, KC>::PB; q++) {
         const int e = threadIdx.x + NT * q, r = e / KC;
         if (e < KC * 16 * NB) g.sb[u][e - r * KC][r] = R.b[u][q];
     }
 }
 
-template <int NB, bool SAME>
+template <int NB, bool SAME, int KC>
 __device__ __forceinline__ void slab_gemm2(f64x4 (&acc1)[(NB + 1) / 2], f64x4 (&acc2)[(NB + 1) / 2], int ni, int r0,
                                            const double *__restrict__ A1, const double *__restrict__ B1,
                                            const double *__restrict__ A2, const double *__restrict__ B2,
-                                           SlabShared<NB> &g) {
+                                           SlabShared<NB, KC> &g) {
     constexpr int TB = (NB + 1) / 2;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int bi = w & 1, wc = w >> 1, lc = l & 15, lk = l >> 4;
-    constexpr int KC = SlabKC<NB>::value;
     const int nch = (ni + KC - 1) / KC;
     const bool two = A2 != nullptr;
-    SlabRegs<NB> R;
-    slab_load<NB>(R, 0, ni, r0, 0, A1, B1);
-    if (two) slab_load<NB>(R, 1, ni, r0, 0, A2, B2);
+    SlabRegs<NB, KC> R;
+    slab_load<NB, KC>(R, 0, ni, r0, 0, A1, B1);
+    if (two) slab_load<NB, KC>(R, 1, ni, r0, 0, A2, B2);
     for (int ch = 0; ch < nch; ch++) {
         __syncthreads();   // the previous chunk's reads of g are done
-        slab_stage<NB>(R, 0, g);
-        if (two) slab_stage<NB>(R, 1, g);
+        slab_stage<NB, KC>(R, 0, g);
+        if (two) slab_stage<NB, KC>(R, 1, g);
         __syncthreads();
         if (ch + 1 < nch) {
-            slab_load<NB>(R, 0, ni, r0, (ch + 1) * KC, A1, B1);
-            if (two) slab_load<NB>(R, 1, ni, r0, (ch + 1) * KC, A2, B2);
+            slab_load<NB, KC>(R, 0, ni, r0, (ch + 1) * KC, A1, B1);
+            if (two) slab_load<NB, KC>(R, 1, ni, r0, (ch + 1) * KC, A2, B2);
         }
 #pragma unroll
         for (int k4 = 0; k4 < KC / 4; k4++) {
