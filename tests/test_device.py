@@ -254,20 +254,23 @@ def test_block_factor_solve_against_dense(ni, nz, variant):
 
 
 @gpu
-def test_cyclic_reduction_ragged_batch_matches_dense():
-    """Cyclic reduction over columns of different heights (1 to 9 levels)."""
+@pytest.mark.parametrize('ni,nzs', [(21, [2, 3, 5, 8, 13, 150, 4]),    # 1 to 7 levels
+                                    (21, [6, 150, 9, 2, 3, 5]),       # odd counts at several levels
+                                    (93, [37, 40]),                   # three row groups per block
+                                    (93, [75, 2])])
+def test_cyclic_reduction_ragged_batch_matches_dense(ni, nzs):
+    """Cyclic reduction over columns of different heights."""
     torch = _torch()
     from vulcan_amd.device import DeviceColumns, DevicePlan, dptr, stream_handle
-    ni = 21
-    nzs = [2, 3, 5, 8, 13, 150, 4]
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(5 + len(nzs) + ni)
     plan = DevicePlan(_sized_network(ni))
     lib = _lib.load()
     _lib.check(lib.vk_plan_set_factor_variant(plan.handle, 4), 'vk_plan_set_factor_variant')
     cols = DeviceColumns(nzs)
     Ds, Ls, Us, bs, xs = [], [], [], [], []
     for nz in nzs:
-        D = rng.standard_normal((nz, ni, ni)) + 6 * np.eye(ni)
+        # the shift keeps the blocks' eigenvalues (a disc of radius ~sqrt(ni)) off zero
+        D = rng.standard_normal((nz, ni, ni)) + 6 * max(1.0, np.sqrt(ni / 21)) * np.eye(ni)
         L = rng.standard_normal((nz, ni)); L[0] = 0
         U = rng.standard_normal((nz, ni)); U[-1] = 0
         b = rng.standard_normal((nz, ni))

@@ -6,14 +6,16 @@ namespace {
 // ---------------------------------------------------------------------------
 // Block cyclic reduction (factor variant 4)
 //
-// Every level eliminates the odd-positioned active layers of each column at
-// once: their blocks are inverted in parallel (blk_invert), and each
-// surviving layer j with eliminated neighbours l (left) and r (right) takes
+// Every level eliminates every other active layer of each column at once
+// (the even positions of an odd count, the odd ones of an even count, so
+// that floor(n/2) layers survive): their blocks are inverted in parallel
+// (blk_invert), and each surviving layer j with eliminated neighbours l
+// (left) and r (right) takes
 //     P_j = B_j A_l^-1,   Q_j = C_j A_r^-1,
 //     A_j -= P_j C_l + Q_j B_r,   B_j = -P_j B_l,   C_j = -Q_j C_r
 // (B: coupling to the previous active layer, C: to the next; diagonal at
-// level 0, dense above).  The chain over layers becomes log2(nz) levels of
-// independent work: the Schur-complement updates above level 0 are dense
+// level 0, dense above).  The chain over layers becomes floor(log2 nz)
+// levels of independent work and one inversion on top: the Schur-complement updates above level 0 are dense
 // n x n products on FP64 MFMA, level 0 is elementwise.  P and Q are kept
 // for the right-hand side reduction of the solves.
 //

@@ -39,9 +39,11 @@ struct vk_plan {
     std::vector<void *> allocs;
 };
 
-// Block cyclic reduction schedule of a set of columns: at level s the
-// active layers of each column at odd positions are eliminated, those at
-// even positions survive; one layer per column remains on top.
+// Block cyclic reduction schedule of a set of columns: at each level every
+// other active layer of a column is eliminated, the rest survive (from an
+// odd count the even positions, both ends included, are eliminated, so
+// floor(n/2) layers survive and a column of nz layers needs floor(log2 nz)
+// levels); one layer per column remains on top.
 struct vk_bcr_sched {
     int nlev = 0;
     std::vector<int> elim_off, surv_off;   // nlev+1 each, into the entry arrays

@@ -529,20 +529,22 @@ static int bcr_schedule(vk_columns *c) {
             const int n = (int)a.size();
             if (n < 2) continue;
             any = true;
-            for (int q = 1; q < n; q += 2) el.push_back(make_int4(a[q], a[q - 1], q + 1 < n ? a[q + 1] : -1, i));
-            for (int q = 0; q < n; q += 2) {
+            // eliminated positions: the even ones of an odd count (floor(n/2)
+            // survivors, one level fewer than keeping the ends), else the odd ones
+            const int pe = n & 1 ? 0 : 1;
+            for (int q = pe; q < n; q += 2)
+                el.push_back(make_int4(a[q], q - 1 >= 0 ? a[q - 1] : -1, q + 1 < n ? a[q + 1] : -1, i));
+            std::vector<int> keep;
+            for (int q = 1 - pe; q < n; q += 2) {
                 const int left = q - 1 >= 0 ? a[q - 1] : -1, right = q + 1 < n ? a[q + 1] : -1;
                 sv.push_back(make_int4(a[q], left, right, i));
-                fl.push_back((left >= 0 ? 1 : 0) | (q + 2 < n ? 2 : 0));
+                // new couplings reach the layers two positions away, where they exist
+                fl.push_back((q - 2 >= 0 ? 1 : 0) | (q + 2 < n ? 2 : 0));
+                keep.push_back(a[q]);
             }
-        }
-        if (!any) break;
-        for (int i = 0; i < c->ncols; i++) {
-            if (act[i].size() < 2) continue;
-            std::vector<int> keep;
-            for (size_t q = 0; q < act[i].size(); q += 2) keep.push_back(act[i][q]);
             act[i] = keep;
         }
+        if (!any) break;
         b.elim_off.push_back((int)el.size());
         b.surv_off.push_back((int)sv.size());
         b.nlev++;
