@@ -201,29 +201,40 @@ __global__ void __launch_bounds__(DECIDE_NT) k_decide(int natom, const int *__re
         __syncthreads();
         pc = dsm;
     }
-    if (threadIdx.x != 0) return;
-    double delta = -1.0;
+    // the column reductions, one thread each (every one in layer order):
+    // 0 the largest delta, 1 any negative entry, 2 / 3 the small and
+    // negative sums, 4 + a the total of atom a
+    __shared__ double red[4 + VK_NATOM_MAX];
+    const int t = threadIdx.x, nq = mode == 0 ? 1 : 4 + natom;
+    if (t < nq) {
+        const int col = t == 0 ? VKP_DELTA : t == 1 ? VKP_NEG : t == 2 ? VKP_SMALL : t == 3 ? VKP_NEGY
+                                                                                      : VKP_ATOM + t - 4;
+        const double *pq = pc + col;
+        // one loop for all lanes (no divergence): each keeps the reduction it needs
+        double sum = 0.0, mx = -1.0;
+        int nz = 0;
 #pragma unroll 8
-    for (int l = l0; l < l1; l++) delta = fmax(delta, pc[(size_t)(l - l0) * NP + VKP_DELTA]);
+        for (int l = 0; l < l1 - l0; l++) {
+            const double v = pq[(size_t)l * NP];
+            sum += v;
+            mx = fmax(mx, v);
+            nz |= v != 0.0;
+        }
+        red[t] = t == 0 ? mx : t == 1 ? (nz ? 1.0 : 0.0) : sum;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double delta = red[0];
     if (delta < 0) delta = 0.0;
     s.delta = delta;
     if (mode == 0) { st[c] = s; return; }
-    int neg = 0;
-    double small = 0.0, nega = 0.0;
-#pragma unroll 8
-    for (int l = l0; l < l1; l++) {
-        neg |= pc[(size_t)(l - l0) * NP + VKP_NEG] != 0.0;
-        small += pc[(size_t)(l - l0) * NP + VKP_SMALL];
-        nega += pc[(size_t)(l - l0) * NP + VKP_NEGY];
-    }
-    s.small_y += fabs(small);
-    s.nega_y += fabs(nega);
+    const bool neg = red[1] != 0.0;
+    s.small_y += fabs(red[2]);
+    s.nega_y += fabs(red[3]);
     double lossmax = 0.0;
     bool lossnan = false;
     for (int a = 0; a < natom; a++) {
-        double acc = 0.0;
-#pragma unroll 8
-        for (int l = l0; l < l1; l++) acc += pc[(size_t)(l - l0) * NP + VKP_ATOM + a];
+        const double acc = red[4 + a];
         const double ini = atom_ini[(size_t)c * natom + a];
         const double loss = (acc - ini) / ini;
         atom_loss[(size_t)c * natom + a] = loss;
