@@ -2,6 +2,8 @@
 // left-hand-side assembly.  One workgroup per layer for the per-layer
 // kernels: the layer's y row and its reaction rates live in LDS, species
 // rows are gathered from per-species CSR lists (no atomics, fixed order).
+#include <algorithm>
+
 #include "vk_internal.h"
 #include "vk_device.h"
 
@@ -322,19 +324,23 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
             double u = 0.0, lo = 0.0;
             if (r.has_u) { u = 0.0 - r.u0; if (r.has_mol) u = u - r.u1; }
             if (r.has_l) { lo = 0.0 - r.l0; if (r.has_mol) lo = lo - r.l1; }
-            Usup[(size_t)l * ni + i] = u;
-            Lsub[(size_t)l * ni + i] = lo;
+            if (blockIdx.y == 0) {
+                Usup[(size_t)l * ni + i] = u;
+                Lsub[(size_t)l * ni + i] = lo;
+            }
         }
     }
     __syncthreads();
     // Every dense entry is written once.  Entries with at most JAC_LIGHT
     // terms (and the structural zeros) in one coalesced pass over the dense
     // index; the few long term lists (jac_ent[0, nheavy): the list is sorted
-    // by term count) by their own threads alongside.
-    for (int q = threadIdx.x; q < nheavy; q += blockDim.x) jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0,
-                                                                     dg1, Dl);
+    // by term count) by their own threads alongside.  With few layers the
+    // entries of a layer are split over the grid's y dimension (each
+    // workgroup forms the layer's shared terms itself).
+    const int e0 = threadIdx.x + blockIdx.y * blockDim.x, es = blockDim.x * gridDim.y;
+    for (int q = e0; q < nheavy; q += es) jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0, dg1, Dl);
 #pragma unroll 2
-    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+    for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
         const int code = jac_emap[e];
         if (code == -2) continue;
@@ -426,7 +432,16 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
         int r = launch_ysum(cols, plan->ni, y, ysum, s);
         if (r) return r;
     }
-    hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
+    // fewer layers than twice the CUs: split each layer's entries over up to 4 workgroups
+    static const int ncu = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        return n;
+    }();
+    const int gy = ncu > 0 ? std::max(1, std::min(4, 2 * ncu / std::max(cols->L, 1))) : 1;
+    hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L, gy), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        plan->njent, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
                        plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
                        dtc, ysum, st, D, Lsub, Usup);
