@@ -104,19 +104,28 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
 }
 
 // Levels >= 1: dense couplings, products on FP64 MFMA, one workgroup per
-// (layer, 32-row slab): a slab's output rows depend only on the same rows
-// of the left operands, so the slabs are independent.  Two launches after
-// the level's inversions, each one product deep:
+// (layer, 32-row slab, half of the output columns): a slab's output rows
+// depend only on the same rows of the left operands, so the slabs are
+// independent, and the column halves split a slab's MFMA chain and right
+// operands over two workgroups (blocks wider than 16 only).  Two launches
+// after the level's inversions, each one product deep:
 //
 // k_bcr_pq, survivors j (x < n):       P_j = B_j A_l^-1,  Q_j = C_j A_r^-1
 //           eliminated k (x >= n):     the back-substitution operators
 //                                      XB_k = A_k^-1 B_k,  XC_k = A_k^-1 C_k
 //           (an eliminated layer's couplings are final: only survivors'
 //           couplings change at this level)
-// k_bcr_update, survivors j, z = 0:    A_j -= P_j C_l + Q_j B_r
-//                            z = 1:    B_j = -P_j B_l,  C_j = -Q_j C_r
-//           (z = 0 reads the eliminated neighbours' couplings, z = 1 writes
-//           the survivor's own: the two slices are independent)
+// k_bcr_update, survivors j, slice 0:    A_j -= P_j C_l + Q_j B_r
+//                            slice 1:    B_j = -P_j B_l,  C_j = -Q_j C_r
+//           (slice 0 reads the eliminated neighbours' couplings, slice 1
+//           writes the survivor's own: the two slices are independent)
+// Grid z: column half (k_bcr_pq), slice * BCR_NCS<NB> + column half
+// (k_bcr_update).
+template <int NB>
+constexpr int BCR_NCS = NB >= 2 ? 2 : 1;            // column halves
+template <int NB>
+constexpr int BCR_NBH = (NB + BCR_NCS<NB> - 1) / BCR_NCS<NB>;   // column blocks per workgroup
+
 template <int NB, int KC>
 __global__ void __launch_bounds__(NT) k_bcr_pq(int ni, int n, const int4 *__restrict__ sv, int slot0,
                                                const double *__restrict__ Dinv, const double *__restrict__ Bt,
@@ -124,32 +133,33 @@ __global__ void __launch_bounds__(NT) k_bcr_pq(int ni, int n, const int4 *__rest
                                                double *__restrict__ Qt, const int4 *__restrict__ el, int ne,
                                                double *__restrict__ XBt, double *__restrict__ XCt,
                                                const vk_colstate *__restrict__ st) {
-    const int q = blockIdx.x, r0 = 32 * blockIdx.y;
-    if (q >= n + ne || r0 >= ni) return;
+    constexpr int NBH = BCR_NBH<NB>;
+    const int q = blockIdx.x, r0 = 32 * blockIdx.y, c0 = 16 * NBH * (int)blockIdx.z;
+    if (q >= n + ne || r0 >= ni || c0 >= ni) return;
     const int4 e = q >= n ? el[q - n] : sv[q];
     if (st && st[e.w].done) return;
-    __shared__ SlabShared<NB, KC> g;
+    __shared__ SlabShared<NBH, KC> g;
     const size_t nn = (size_t)ni * ni;
-    f64x4 acc[(NB + 1) / 2], acc2[(NB + 1) / 2];
-    slab_zero<NB>(acc);
-    slab_zero<NB>(acc2);
+    f64x4 acc[(NBH + 1) / 2], acc2[(NBH + 1) / 2];
+    slab_zero<NBH>(acc);
+    slab_zero<NBH>(acc2);
     if (q >= n) {
         const size_t k = e.x;
         // A_k^-1 (a, c) = X[c][a]: the transposed-storage inverse is the At operand
-        slab_gemm2<NB, false, KC>(acc, acc2, ni, r0, Dinv + k * nn, Bt + k * nn, Dinv + k * nn, Ct + k * nn, g);
-        slab_store<NB>(acc, ni, r0, XBt + k * nn, true, 1.0, nullptr);
-        slab_store<NB>(acc2, ni, r0, XCt + k * nn, true, 1.0, nullptr);
+        slab_gemm2<NBH, false, KC>(acc, acc2, ni, r0, Dinv + k * nn, Bt + k * nn, Dinv + k * nn, Ct + k * nn, g, c0);
+        slab_store<NBH>(acc, ni, r0, XBt + k * nn, true, 1.0, nullptr, c0);
+        slab_store<NBH>(acc2, ni, r0, XCt + k * nn, true, 1.0, nullptr, c0);
         return;
     }
     const size_t j = e.x, slot = (size_t)(slot0 + q) * nn;
     const bool hl = e.y >= 0, hr = e.z >= 0;
     // B_j(a, c) = Bt_j[c][a], inv(c, b) = X[b][c]
-    if (hl) slab_gemm2<NB, false, KC>(acc, acc2, ni, r0, Bt + j * nn, Dinv + (size_t)e.y * nn,
-                                      hr ? Ct + j * nn : nullptr, hr ? Dinv + (size_t)e.z * nn : nullptr, g);
-    else if (hr) slab_gemm2<NB, false, KC>(acc2, acc, ni, r0, Ct + j * nn, Dinv + (size_t)e.z * nn, nullptr, nullptr,
-                                           g);
-    if (hl) slab_store<NB>(acc, ni, r0, Pt + slot, true, 1.0, nullptr);
-    if (hr) slab_store<NB>(acc2, ni, r0, Qt + slot, true, 1.0, nullptr);
+    if (hl) slab_gemm2<NBH, false, KC>(acc, acc2, ni, r0, Bt + j * nn, Dinv + (size_t)e.y * nn,
+                                      hr ? Ct + j * nn : nullptr, hr ? Dinv + (size_t)e.z * nn : nullptr, g, c0);
+    else if (hr) slab_gemm2<NBH, false, KC>(acc2, acc, ni, r0, Ct + j * nn, Dinv + (size_t)e.z * nn, nullptr, nullptr,
+                                           g, c0);
+    if (hl) slab_store<NBH>(acc, ni, r0, Pt + slot, true, 1.0, nullptr, c0);
+    if (hr) slab_store<NBH>(acc2, ni, r0, Qt + slot, true, 1.0, nullptr, c0);
 }
 
 template <int NB, int KC>
@@ -158,33 +168,35 @@ __global__ void __launch_bounds__(NT) k_bcr_update(int ni, int n, const int4 *__
                                                    double *__restrict__ Bt, double *__restrict__ Ct,
                                                    const double *__restrict__ Pt, const double *__restrict__ Qt,
                                                    const vk_colstate *__restrict__ st) {
-    const int q = blockIdx.x, r0 = 32 * blockIdx.y;
-    if (q >= n || r0 >= ni) return;
+    constexpr int NBH = BCR_NBH<NB>;
+    const int q = blockIdx.x, r0 = 32 * blockIdx.y, half = (int)blockIdx.z % BCR_NCS<NB>;
+    const int slice = (int)blockIdx.z / BCR_NCS<NB>, c0 = 16 * NBH * half;
+    if (q >= n || r0 >= ni || c0 >= ni) return;
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
-    __shared__ SlabShared<NB, KC> g;
+    __shared__ SlabShared<NBH, KC> g;
     const size_t nn = (size_t)ni * ni, j = e.x;
     const double *Ptj = Pt + (size_t)(slot0 + q) * nn, *Qtj = Qt + (size_t)(slot0 + q) * nn;
-    f64x4 acc[(NB + 1) / 2], acc2[(NB + 1) / 2];
-    slab_zero<NB>(acc);
-    slab_zero<NB>(acc2);
-    if (blockIdx.z == 0) {
+    f64x4 acc[(NBH + 1) / 2], acc2[(NBH + 1) / 2];
+    slab_zero<NBH>(acc);
+    slab_zero<NBH>(acc2);
+    if (slice == 0) {
         // one accumulator for both products
         const bool hl = e.y >= 0, hr = e.z >= 0;
-        if (hl) slab_gemm2<NB, true, KC>(acc, acc2, ni, r0, Ptj, Ct + (size_t)e.y * nn, hr ? Qtj : nullptr,
-                                         hr ? Bt + (size_t)e.z * nn : nullptr, g);
-        else if (hr) slab_gemm2<NB, true, KC>(acc, acc2, ni, r0, Qtj, Bt + (size_t)e.z * nn, nullptr, nullptr, g);
-        slab_store<NB>(acc, ni, r0, A + j * nn, false, -1.0, A + j * nn);
+        if (hl) slab_gemm2<NBH, true, KC>(acc, acc2, ni, r0, Ptj, Ct + (size_t)e.y * nn, hr ? Qtj : nullptr,
+                                         hr ? Bt + (size_t)e.z * nn : nullptr, g, c0);
+        else if (hr) slab_gemm2<NBH, true, KC>(acc, acc2, ni, r0, Qtj, Bt + (size_t)e.z * nn, nullptr, nullptr, g, c0);
+        slab_store<NBH>(acc, ni, r0, A + j * nn, false, -1.0, A + j * nn, c0);
         return;
     }
     // the new couplings reach past the eliminated neighbours (zero where there is no layer beyond)
     const int fl = sflag[q];
     const bool nl = fl & 1, nr = fl & 2;
-    if (nl) slab_gemm2<NB, false, KC>(acc, acc2, ni, r0, Ptj, Bt + (size_t)e.y * nn, nr ? Qtj : nullptr,
-                                      nr ? Ct + (size_t)e.z * nn : nullptr, g);
-    else if (nr) slab_gemm2<NB, false, KC>(acc2, acc, ni, r0, Qtj, Ct + (size_t)e.z * nn, nullptr, nullptr, g);
-    slab_store<NB>(acc, ni, r0, Bt + j * nn, true, -1.0, nullptr);
-    slab_store<NB>(acc2, ni, r0, Ct + j * nn, true, -1.0, nullptr);
+    if (nl) slab_gemm2<NBH, false, KC>(acc, acc2, ni, r0, Ptj, Bt + (size_t)e.y * nn, nr ? Qtj : nullptr,
+                                      nr ? Ct + (size_t)e.z * nn : nullptr, g, c0);
+    else if (nr) slab_gemm2<NBH, false, KC>(acc2, acc, ni, r0, Qtj, Ct + (size_t)e.z * nn, nullptr, nullptr, g, c0);
+    slab_store<NBH>(acc, ni, r0, Bt + j * nn, true, -1.0, nullptr, c0);
+    slab_store<NBH>(acc2, ni, r0, Ct + j * nn, true, -1.0, nullptr, c0);
 }
 
 __global__ void k_copy_d(long n, const double *__restrict__ a, double *__restrict__ b) {
@@ -368,15 +380,14 @@ template <int NB, int KC>
 void launch_bcr_products(int ni, int ns, int ne, int ny, const int4 *sv, const int *sflag, int slot0,
                          const int4 *el, const double *Dinv, const vk_bcr_ws *ws, const vk_colstate *st,
                          hipStream_t s) {
-    hipLaunchKernelGGL((k_bcr_pq<NB, KC>), dim3(ns + ne, ny), dim3(NT), 0, s, ni, ns, sv, slot0, Dinv, ws->B, ws->C,
-                       ws->P, ws->Q, el, ne, ws->XB, ws->XC, st);
-    hipLaunchKernelGGL((k_bcr_update<NB, KC>), dim3(ns, ny, 2), dim3(NT), 0, s, ni, ns, sv, sflag, slot0, ws->A,
-                       ws->B, ws->C, ws->P, ws->Q, st);
+    hipLaunchKernelGGL((k_bcr_pq<NB, KC>), dim3(ns + ne, ny, BCR_NCS<NB>), dim3(NT), 0, s, ni, ns, sv, slot0, Dinv,
+                       ws->B, ws->C, ws->P, ws->Q, el, ne, ws->XB, ws->XC, st);
+    hipLaunchKernelGGL((k_bcr_update<NB, KC>), dim3(ns, ny, 2 * BCR_NCS<NB>), dim3(NT), 0, s, ni, ns, sv, sflag, slot0,
+                       ws->A, ws->B, ws->C, ws->P, ws->Q, st);
 }
 
-// compute units of the current device, queried once per device (deep
-// product chunks take one workgroup's LDS per CU: used where a launch fits
-// the CUs); 0 if the query fails
+// compute units of the current device, queried once per device; 0 if the
+// query fails
 int device_cus() {
     static int cus[64];
     int dev = 0;
@@ -392,7 +403,10 @@ template <int NB>
 int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const double *Lsub, const double *Usup,
                         double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
     const vk_bcr_sched &b = cols->bcr;
-    const long ncu = device_cus();
+    // deep chunks where every workgroup of a launch is resident at once
+    // (their LDS admits two workgroups per CU up to 80 KiB)
+    constexpr long per_cu = sizeof(SlabShared<BCR_NBH<NB>, SlabKC<NB>::deep>) <= 80 * 1024 ? 2 : 1;
+    const long cap = per_cu * device_cus();
     const int ny = (ni + 31) / 32;
     const long nd = (long)cols->L * ni * ni;
     hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, nd, D, ws->A);
@@ -407,7 +421,7 @@ int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const d
                 hipLaunchKernelGGL(k_bcr_update0, dim3(ns + ne, 8), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
                                    b.d_sflag + s0, s0, Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q,
                                    b.d_elim + e0, ne, ws->XB, ws->XC, st);
-            else if ((long)(ns + ne) * ny <= ncu && 2L * ns * ny <= ncu)
+            else if ((long)(ns + ne) * ny * BCR_NCS<NB> <= cap && 2L * ns * ny * BCR_NCS<NB> <= cap)
                 launch_bcr_products<NB, SlabKC<NB>::deep>(ni, ns, ne, ny, b.d_surv + s0, b.d_sflag + s0, s0,
                                                           b.d_elim + e0, Dinv, ws, st, s);
             else

@@ -290,9 +290,9 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
     }
 }
 
-// Products for the cyclic-reduction updates, one 32-row output slab per
-// workgroup, two products at a time:
-//     acc1 += A1[r0:r0+32, :] B1,   acc2 (or acc1 if SAME) += A2[r0:r0+32, :] B2
+// Products for the cyclic-reduction updates, one 32-row output slab (and
+// the 16 NB output columns from c0) per workgroup, two products at a time:
+//     acc1 += A1[r0:r0+32, :] B1[:, c0:],   acc2 (or acc1 if SAME) += A2[r0:r0+32, :] B2[:, c0:]
 // over k < ni, with every operand stored transposed:
 //     A(a, c) = At[c*ni + a],  B(c, b) = Bt[b*ni + c].
 // A null A2 skips the second product.  Output tiles are 16 x 16 (MFMA C
@@ -302,9 +302,10 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
 // are issued into registers before the current chunk's MFMAs, and each
 // chunk is read with the operand's unit-stride index fastest across threads
 // (A: the row a, B: the inner index c).  Each chunk costs about one global
-// load latency, so deep chunks suit launches that fit the CUs one workgroup
-// each, shallow ones keep two or three workgroups per CU resident.
-// Two staged operand pairs take 16 KC (34 + 16 NB) bytes of LDS.
+// load latency, so deep chunks suit launches whose workgroups are all
+// resident at once, shallow ones keep more workgroups per CU resident.
+// Two staged operand pairs take 16 KC (34 + 16 NB) bytes of LDS (NB: the
+// slab's column blocks); SlabKC is indexed by the block's column blocks.
 template <int NB>
 struct SlabKC {
     static constexpr int shallow = NB <= 7 ? 24 : 16;   // <= 56 KiB
@@ -326,7 +327,7 @@ struct SlabRegs {
 
 template <int NB, int KC>
 __device__ __forceinline__ void slab_load(SlabRegs<NB, KC> &R, int u, int ni, int r0, int k0,
-                                          const double *__restrict__ At, const double *__restrict__ Bt) {
+                                          const double *__restrict__ At, const double *__restrict__ Bt, int c0) {
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB, KC>::PA; q++) {
         const int e = threadIdx.x + NT * q, k = e >> 5, r = e & 31;   // a = r0 + r fastest
@@ -336,8 +337,8 @@ __device__ __forceinline__ void slab_load(SlabRegs<NB, KC> &R, int u, int ni, in
 #pragma unroll
     for (int q = 0; q < SlabRegs<NB, KC>::PB; q++) {
         const int e = threadIdx.x + NT * q, r = e / KC, k = e - r * KC;   // c = k0 + k fastest
-        const int c = k0 + k;
-        R.b[u][q] = (e < KC * 16 * NB && r < ni && c < ni) ? Bt[(size_t)r * ni + c] : 0.0;
+        const int c = k0 + k, b = c0 + r;
+        R.b[u][q] = (e < KC * 16 * NB && b < ni && c < ni) ? Bt[(size_t)b * ni + c] : 0.0;
     }
 }
 
@@ -359,23 +360,23 @@ template <int NB, bool SAME, int KC>
 __device__ __forceinline__ void slab_gemm2(f64x4 (&acc1)[(NB + 1) / 2], f64x4 (&acc2)[(NB + 1) / 2], int ni, int r0,
                                            const double *__restrict__ A1, const double *__restrict__ B1,
                                            const double *__restrict__ A2, const double *__restrict__ B2,
-                                           SlabShared<NB, KC> &g) {
+                                           SlabShared<NB, KC> &g, int c0) {
     constexpr int TB = (NB + 1) / 2;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int bi = w & 1, wc = w >> 1, lc = l & 15, lk = l >> 4;
     const int nch = (ni + KC - 1) / KC;
     const bool two = A2 != nullptr;
     SlabRegs<NB, KC> R;
-    slab_load<NB, KC>(R, 0, ni, r0, 0, A1, B1);
-    if (two) slab_load<NB, KC>(R, 1, ni, r0, 0, A2, B2);
+    slab_load<NB, KC>(R, 0, ni, r0, 0, A1, B1, c0);
+    if (two) slab_load<NB, KC>(R, 1, ni, r0, 0, A2, B2, c0);
     for (int ch = 0; ch < nch; ch++) {
         __syncthreads();   // the previous chunk's reads of g are done
         slab_stage<NB, KC>(R, 0, g);
         if (two) slab_stage<NB, KC>(R, 1, g);
         __syncthreads();
         if (ch + 1 < nch) {
-            slab_load<NB, KC>(R, 0, ni, r0, (ch + 1) * KC, A1, B1);
-            if (two) slab_load<NB, KC>(R, 1, ni, r0, (ch + 1) * KC, A2, B2);
+            slab_load<NB, KC>(R, 0, ni, r0, (ch + 1) * KC, A1, B1, c0);
+            if (two) slab_load<NB, KC>(R, 1, ni, r0, (ch + 1) * KC, A2, B2, c0);
         }
 #pragma unroll
         for (int k4 = 0; k4 < KC / 4; k4++) {
@@ -412,16 +413,17 @@ __device__ __forceinline__ void slab_zero(f64x4 (&acc)[(NB + 1) / 2]) {
     for (int tj = 0; tj < (NB + 1) / 2; tj++) acc[tj] = f64x4{0.0, 0.0, 0.0, 0.0};
 }
 
-// out(i, c) = sgn * acc (+ add(i, c) if add) for the slab rows i, ni x ni,
-// stored row-major (out[i*ni + c]) or transposed (out[c*ni + i])
+// out(i, c) = sgn * acc (+ add(i, c) if add) for the slab rows i and the
+// columns c0 .. c0 + 16 NB - 1, ni x ni, stored row-major (out[i*ni + c])
+// or transposed (out[c*ni + i])
 template <int NB>
 __device__ __forceinline__ void slab_store(const f64x4 (&acc)[(NB + 1) / 2], int ni, int r0, double *__restrict__ out,
-                                           bool transposed, double sgn, const double *add) {
+                                           bool transposed, double sgn, const double *add, int c0) {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int bi = w & 1, wc = w >> 1, lg = l >> 4, lc = l & 15;
 #pragma unroll
     for (int tj = 0; tj < (NB + 1) / 2; tj++) {
-        const int bj = wc + 2 * tj, cc = 16 * bj + lc;
+        const int bj = wc + 2 * tj, cc = c0 + 16 * bj + lc;
         if (bj >= NB || cc >= ni) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
