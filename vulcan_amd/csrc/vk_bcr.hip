@@ -104,10 +104,10 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
 }
 
 // Levels >= 1: dense couplings, products on FP64 MFMA, one workgroup per
-// (layer, 32-row slab, half of the output columns): a slab's output rows
-// depend only on the same rows of the left operands, so the slabs are
-// independent, and the column halves split a slab's MFMA chain and right
-// operands over two workgroups (blocks wider than 16 only).  Two launches
+// (layer, 32-row slab, 32 output columns): a slab's output rows depend
+// only on the same rows of the left operands, so the slabs are
+// independent, and the column groups split a slab's MFMA chain and right
+// operands over several workgroups.  Two launches
 // after the level's inversions, each one product deep:
 //
 // k_bcr_pq, survivors j (x < n):       P_j = B_j A_l^-1,  Q_j = C_j A_r^-1
@@ -119,12 +119,12 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
 //                            slice 1:    B_j = -P_j B_l,  C_j = -Q_j C_r
 //           (slice 0 reads the eliminated neighbours' couplings, slice 1
 //           writes the survivor's own: the two slices are independent)
-// Grid z: column half (k_bcr_pq), slice * BCR_NCS<NB> + column half
+// Grid z: column group (k_bcr_pq), slice * BCR_NCS<NB> + column group
 // (k_bcr_update).
 template <int NB>
-constexpr int BCR_NCS = NB >= 2 ? 2 : 1;            // column halves
+constexpr int BCR_NBH = NB >= 2 ? 2 : 1;                        // column blocks per workgroup
 template <int NB>
-constexpr int BCR_NBH = (NB + BCR_NCS<NB> - 1) / BCR_NCS<NB>;   // column blocks per workgroup
+constexpr int BCR_NCS = (NB + BCR_NBH<NB> - 1) / BCR_NBH<NB>;   // workgroups across a block's columns
 
 template <int NB, int KC>
 __global__ void __launch_bounds__(NT) k_bcr_pq(int ni, int n, const int4 *__restrict__ sv, int slot0,
@@ -169,8 +169,8 @@ __global__ void __launch_bounds__(NT) k_bcr_update(int ni, int n, const int4 *__
                                                    const double *__restrict__ Pt, const double *__restrict__ Qt,
                                                    const vk_colstate *__restrict__ st) {
     constexpr int NBH = BCR_NBH<NB>;
-    const int q = blockIdx.x, r0 = 32 * blockIdx.y, half = (int)blockIdx.z % BCR_NCS<NB>;
-    const int slice = (int)blockIdx.z / BCR_NCS<NB>, c0 = 16 * NBH * half;
+    const int q = blockIdx.x, r0 = 32 * blockIdx.y, cg = (int)blockIdx.z % BCR_NCS<NB>;
+    const int slice = (int)blockIdx.z / BCR_NCS<NB>, c0 = 16 * NBH * cg;
     if (q >= n || r0 >= ni || c0 >= ni) return;
     const int4 e = sv[q];
     if (st && st[e.w].done) return;
@@ -403,8 +403,8 @@ template <int NB>
 int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const double *Lsub, const double *Usup,
                         double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
     const vk_bcr_sched &b = cols->bcr;
-    // deep chunks where every workgroup of a launch is resident at once
-    // (their LDS admits two workgroups per CU up to 80 KiB)
+    // deep chunks where every workgroup of a launch is resident at once,
+    // at most two per CU (with three, measured slower than shallow chunks)
     constexpr long per_cu = sizeof(SlabShared<BCR_NBH<NB>, SlabKC<NB>::deep>) <= 80 * 1024 ? 2 : 1;
     const long cap = per_cu * device_cus();
     const int ny = (ni + 31) / 32;
