@@ -1,4 +1,6 @@
 // Block cyclic reduction factorisation and solves (factor variant 4).
+#include <atomic>
+
 #include "vk_blk.h"
 
 namespace {
@@ -389,14 +391,15 @@ void launch_bcr_products(int ni, int ns, int ne, int ny, const int4 *sv, const i
 // compute units of the current device, queried once per device; 0 if the
 // query fails
 int device_cus() {
-    static int cus[64];
+    static std::atomic<int> cus[64];   // zero-initialised (static storage)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-    if (cus[dev] <= 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus[dev] = n;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 0;
+        cus[dev].store(n, std::memory_order_relaxed);
     }
-    return cus[dev];
+    return n;
 }
 
 template <int NB>
