@@ -1,4 +1,5 @@
 // Block cyclic reduction factorisation and solves (factor variant 4).
+#include <algorithm>
 #include <atomic>
 
 #include "vk_blk.h"
@@ -21,7 +22,8 @@ namespace {
 // n x n products on FP64 MFMA, level 0 is elementwise.  P and Q are kept
 // for the right-hand side reduction of the solves.
 //
-// Storage: A row-major (it is inverted in place of D); P, Q, B, C
+// Storage: A row-major (the survivors' Schur complements, formed from D by
+// the level-0 update; level 0 inverts from D itself); P, Q, B, C
 // transposed (Pt[b][a] = P[a][b]), so that the solve sweeps, one thread per
 // row a, read them coalesced; inverses transposed as in the chain variants.
 // ---------------------------------------------------------------------------
@@ -46,17 +48,22 @@ __global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 
     if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
 }
 
-// level 0: couplings are the diagonals Lsub (B) and Usup (C); elementwise,
-// over the transposed index tt = b*ni + a (coalesced P, Q, B, C and
-// inverses), each block's index range split over the grid's y dimension
+// level 0: couplings are the diagonals Lsub (B) and Usup (C); elementwise.
+// Survivors: 32 x 32 tiles (a: block row, b: block column), the grid's y
+// dimension over a block's tiles; P, Q, B, C (transposed storage, index
+// tt = b*ni + a) are written in the loop order with a fastest, the Schur
+// update of A (row-major) goes through LDS and is written with b fastest:
+// A = D - update (A holds nothing before level 0: the level's eliminated
+// blocks are inverted from D directly).
 // Workgroups n .. n+ne-1 form the back-substitution operators of the
-// eliminated layers instead: XB = A_k^-1 diag(Lsub_k), XC = A_k^-1 diag(Usup_k).
+// eliminated layers instead: XB = A_k^-1 diag(Lsub_k), XC = A_k^-1 diag(Usup_k)
+// (over tt, the block's range split over the grid's y dimension).
 __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const int *__restrict__ sflag,
                               int slot0, const double *__restrict__ Lsub, const double *__restrict__ Usup,
-                              const double *__restrict__ Dinv, double *__restrict__ A, double *__restrict__ Bt,
-                              double *__restrict__ Ct, double *__restrict__ Pt, double *__restrict__ Qt,
-                              const int4 *__restrict__ el, int ne, double *__restrict__ XBt,
-                              double *__restrict__ XCt, const vk_colstate *__restrict__ st) {
+                              const double *__restrict__ Dinv, const double *__restrict__ D, double *__restrict__ A,
+                              double *__restrict__ Bt, double *__restrict__ Ct, double *__restrict__ Pt,
+                              double *__restrict__ Qt, const int4 *__restrict__ el, int ne,
+                              double *__restrict__ XBt, double *__restrict__ XCt, const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x;
     if (q >= n + ne) return;
     if (q >= n) {
@@ -80,28 +87,43 @@ __global__ void k_bcr_update0(int ni, int n, const int4 *__restrict__ sv, const 
     const size_t slot = (size_t)(slot0 + q) * nn;
     const double *XL = e.y >= 0 ? Dinv + (size_t)e.y * nn : nullptr;   // X[m][i] = inv[i][m]
     const double *XR = e.z >= 0 ? Dinv + (size_t)e.z * nn : nullptr;
-    const int span = (ni * ni + gridDim.y - 1) / gridDim.y, t0 = blockIdx.y * span;
-    const int t1 = min(ni * ni, t0 + span);
-    for (int tt = t0 + threadIdx.x; tt < t1; tt += blockDim.x) {
-        const int b = tt / ni, a = tt - b * ni;
-        double upd = 0.0;
-        if (XL) {
-            const double p = Lsub[j * ni + a] * XL[tt];
-            Pt[slot + tt] = p;
-            upd = p * Usup[(size_t)e.y * ni + b];
-            Bt[j * nn + tt] = (fl & 1) ? -(p * Lsub[(size_t)e.y * ni + b]) : 0.0;
-        } else {
-            Bt[j * nn + tt] = 0.0;
+    __shared__ double up[32][33];
+    const int tpr = (ni + 31) / 32;
+    for (int tile = blockIdx.y; tile < tpr * tpr; tile += gridDim.y) {
+        const int a0 = 32 * (tile % tpr), b0 = 32 * (tile / tpr);
+        for (int u = threadIdx.x; u < 32 * 32; u += blockDim.x) {
+            const int al = u & 31, bl = u >> 5, a = a0 + al, b = b0 + bl;
+            double upd = 0.0;
+            if (a < ni && b < ni) {
+                const size_t tt = (size_t)b * ni + a;
+                if (XL) {
+                    const double p = Lsub[j * ni + a] * XL[tt];
+                    Pt[slot + tt] = p;
+                    upd = p * Usup[(size_t)e.y * ni + b];
+                    Bt[j * nn + tt] = (fl & 1) ? -(p * Lsub[(size_t)e.y * ni + b]) : 0.0;
+                } else {
+                    Bt[j * nn + tt] = 0.0;
+                }
+                if (XR) {
+                    const double qv = Usup[j * ni + a] * XR[tt];
+                    Qt[slot + tt] = qv;
+                    upd = upd + qv * Lsub[(size_t)e.z * ni + b];
+                    Ct[j * nn + tt] = (fl & 2) ? -(qv * Usup[(size_t)e.z * ni + b]) : 0.0;
+                } else {
+                    Ct[j * nn + tt] = 0.0;
+                }
+            }
+            up[al][bl] = upd;
         }
-        if (XR) {
-            const double qv = Usup[j * ni + a] * XR[tt];
-            Qt[slot + tt] = qv;
-            upd = upd + qv * Lsub[(size_t)e.z * ni + b];
-            Ct[j * nn + tt] = (fl & 2) ? -(qv * Usup[(size_t)e.z * ni + b]) : 0.0;
-        } else {
-            Ct[j * nn + tt] = 0.0;
+        __syncthreads();
+        for (int u = threadIdx.x; u < 32 * 32; u += blockDim.x) {
+            const int bl = u & 31, al = u >> 5, a = a0 + al, b = b0 + bl;
+            if (a < ni && b < ni) {
+                const size_t o = j * nn + (size_t)a * ni + b;
+                A[o] = D[o] - up[al][bl];
+            }
         }
-        A[j * nn + (size_t)a * ni + b] = A[j * nn + (size_t)a * ni + b] - upd;
+        __syncthreads();
     }
 }
 
@@ -411,19 +433,19 @@ int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const d
     constexpr long per_cu = sizeof(SlabShared<BCR_NBH<NB>, SlabKC<NB>::deep>) <= 80 * 1024 ? 2 : 1;
     const long cap = per_cu * device_cus();
     const int ny = (ni + 31) / 32;
-    const long nd = (long)cols->L * ni * ni;
-    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, nd, D, ws->A);
+    const int tpr = (ni + 31) / 32;
     for (int lev = 0; lev < b.nlev; lev++) {
         const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
+        // level 0 inverts its eliminated blocks from D (the level-0 update writes A)
         if (ne > 0)
             hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(ne), dim3(NT), 0, s, ni, ne, b.d_elim + e0,
-                               (const int2 *)nullptr, cols->d_off, ws->A, Dinv, info, st);
+                               (const int2 *)nullptr, cols->d_off, lev == 0 ? D : ws->A, Dinv, info, st);
         if (ns > 0) {
             if (lev == 0)
-                hipLaunchKernelGGL(k_bcr_update0, dim3(ns + ne, 8), dim3(NT), 0, s, ni, ns, b.d_surv + s0,
-                                   b.d_sflag + s0, s0, Lsub, Usup, Dinv, ws->A, ws->B, ws->C, ws->P, ws->Q,
-                                   b.d_elim + e0, ne, ws->XB, ws->XC, st);
+                hipLaunchKernelGGL(k_bcr_update0, dim3(ns + ne, std::min(tpr * tpr, 16)), dim3(NT), 0, s, ni, ns,
+                                   b.d_surv + s0, b.d_sflag + s0, s0, Lsub, Usup, Dinv, D, ws->A, ws->B, ws->C,
+                                   ws->P, ws->Q, b.d_elim + e0, ne, ws->XB, ws->XC, st);
             else if ((long)(ns + ne) * ny * BCR_NCS<NB> <= cap && 2L * ns * ny * BCR_NCS<NB> <= cap)
                 launch_bcr_products<NB, SlabKC<NB>::deep>(ni, ns, ne, ny, b.d_surv + s0, b.d_sflag + s0, s0,
                                                           b.d_elim + e0, Dinv, ws, st, s);
