@@ -336,9 +336,15 @@ __device__ __forceinline__ double mv_dot(const double (&m)[MV_CH], const double 
     return s0 + s1;
 }
 
+// rin: the right-hand sides read (the solve's input at level 0, else r
+// itself); at level 0 (sflag non-null) the sweep also places the
+// eliminated layers' right-hand sides into r, each layer once: by its
+// right survivor, or by its left one where it ends the column (no layer
+// beyond it: sflag bit 1 clear).
 __global__ void __launch_bounds__(MV_NT) k_bcr_fwd_s(int ni, int n, const int4 *__restrict__ sv, int slot0,
                                                      const double *__restrict__ P, const double *__restrict__ Q,
-                                                     double *__restrict__ r, const vk_colstate *__restrict__ st) {
+                                                     const double *rin, double *r, const int *__restrict__ sflag,
+                                                     const vk_colstate *__restrict__ st) {
     const int q = blockIdx.x, a0 = MV_RG * blockIdx.y;
     if (q >= n || a0 >= ni) return;
     const int4 e = sv[q];
@@ -350,8 +356,8 @@ __global__ void __launch_bounds__(MV_NT) k_bcr_fwd_s(int ni, int n, const int4 *
     mv_load(mq, Q + (size_t)(slot0 + q) * nn, e.z >= 0 ? ni : 0, a, h, ch);
     __shared__ double rl[MV_NIMAX], rr[MV_NIMAX], pp[MV_SP][MV_RG];
     for (int i = threadIdx.x; i < ni; i += MV_NT) {
-        rl[i] = e.y >= 0 ? r[(size_t)e.y * ni + i] : 0.0;
-        rr[i] = e.z >= 0 ? r[(size_t)e.z * ni + i] : 0.0;
+        rl[i] = e.y >= 0 ? rin[(size_t)e.y * ni + i] : 0.0;
+        rr[i] = e.z >= 0 ? rin[(size_t)e.z * ni + i] : 0.0;
     }
     __syncthreads();
     pp[h][lr] = mv_dot(mp, rl, ni, h, ch) + mv_dot(mq, rr, ni, h, ch);
@@ -360,7 +366,11 @@ __global__ void __launch_bounds__(MV_NT) k_bcr_fwd_s(int ni, int n, const int4 *
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < MV_SP; k++) t += pp[k][threadIdx.x];
-        r[(size_t)e.x * ni + a] = r[(size_t)e.x * ni + a] - t;
+        r[(size_t)e.x * ni + a] = rin[(size_t)e.x * ni + a] - t;
+        if (sflag) {
+            if (e.y >= 0) r[(size_t)e.y * ni + a] = rl[a];
+            if (e.z >= 0 && !(sflag[q] & 2)) r[(size_t)e.z * ni + a] = rr[a];
+        }
     }
 }
 
@@ -485,16 +495,22 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
                         hipStream_t s) {
     const vk_bcr_sched &b = cols->bcr;
     const long n = (long)cols->L * ni;
-    hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
-    // ni <= 128: the row-group sweeps (several workgroups per block)
+    // ni <= 128: the row-group sweeps (several workgroups per block); their
+    // level 0 reads `in` and fills tmp itself, the generic sweeps start
+    // from a copy
     const bool small = ni <= MV_NIMAX;
-    auto fwd = small ? k_bcr_fwd_s : k_bcr_fwd;
     auto bwd = small ? k_bcr_bwd_s : k_bcr_bwd;
     const int bt = small ? MV_NT : BCR_ST, gy = small ? (ni + MV_RG - 1) / MV_RG : 1;
+    if (!small) hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
     for (int lev = 0; lev < b.nlev; lev++) {
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
-        if (ns > 0)
-            hipLaunchKernelGGL(fwd, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp, st);
+        if (ns <= 0) continue;
+        if (small)
+            hipLaunchKernelGGL(k_bcr_fwd_s, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q,
+                               lev == 0 ? in : tmp, tmp, lev == 0 ? b.d_sflag + s0 : (const int *)nullptr, st);
+        else
+            hipLaunchKernelGGL(k_bcr_fwd, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp,
+                               st);
     }
     hipLaunchKernelGGL(bwd, dim3(b.ntop, gy), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, Dinv,
                        ws->XB, ws->XC, tmp, out, st);
