@@ -45,7 +45,7 @@ FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
 # one eager attempt (python3 -m vulcan_amd.profile_kernels under
 # rocprofv3 --pmc, one counter per run), each counter scaled by its reading
 # for a 512 MiB device copy.  Algorithmic: 2 ni^2 x 8 B per layer = 20.8 MB.
-FACTOR_TRAFFIC = {4: 343274577}   # block cyclic reduction: 343.3 MB (16.5x; slab products re-read their operands)
+FACTOR_TRAFFIC = {4: 343401327}   # block cyclic reduction: 343.4 MB (16.5x; slab products re-read their operands)
 
 
 def _sync():
