@@ -35,6 +35,9 @@ extern "C" {
 #define VK_ENOMEM   (-1002)   /* device allocation failed        */
 #define VK_ETOOBIG  (-1003)   /* ni beyond the compiled maximum  */
 
+#define VK_NI_LIMIT 160           /* largest ni the kernels are compiled for */
+#define VK_CONV_IGNORE_WORDS 5    /* ceil(VK_NI_LIMIT / 32)                   */
+
 /* rate-expression kinds (par[p][0..7] = A n E Ainf ninf Einf Fc J0) */
 #define VK_KIND_ARRH   0  /* op.py:166               */
 #define VK_KIND_LIND   1  /* op.py:176-182           */
@@ -113,11 +116,15 @@ typedef struct {
     int use_conv;           /* evaluate the steady-state test (needs history)  */
     int use_update_mu;      /* run update_mu_dz every update_frq steps         */
     int refine;             /* refinement steps per linear solve (1 = default) */
+    /* conver_ignore (vulcan_cfg.py:153, op.py:1045-1046): bit (i % 32) of
+     * word i / 32 set = species i is left out of conv()'s longdy */
+    unsigned int conv_ignore[VK_CONV_IGNORE_WORDS];
 } vk_params;
 
 /* Per-column integrator state (device resident; mirrors var/para fields). */
 typedef struct {
-    double t, dt, delta, longdy, longdydt, small_y, nega_y, pad0;
+    double t, dt, delta, longdy, longdydt, small_y, nega_y;
+    double aflux_change;  /* actinic-flux change tested by conv() (op.py:1060) */
     int count, attempts, nega_count, loss_count, delta_count, end_case;
     int retry;      /* 1: the next attempt retries the current step       */
     int done;       /* 1: stop() returned true, column no longer steps     */
@@ -239,6 +246,26 @@ int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom,
                  double *d_ymix, double *d_y_prev, double *d_atom_ini,
                  double *d_atom_loss, double *d_atom_loss_prev,
                  vk_colstate *d_st, int n_attempts, void *stream);
+/* Integration.stop() with conv() (op.py:1018-1085) for every column that is
+ * not done: sets st.longdy / st.longdydt, and end_case / done when the
+ * column converged (1), ran past runtime (2) or past count_max (3).
+ * History as vk_integrate keeps it: y_time[q] (the state after step q+1)
+ * at d_yhist + (q % H) * L * ni for q >= max(count - conv_step, 0) and
+ * t_time[q] at d_thist[c * tcap + q]; d_y must be y_time[count-1].
+ * With prm->use_conv = 0 only the runtime and count limits are tested. */
+int vk_conv(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+            const vk_params *prm, const double *d_y, const double *d_ymix,
+            const double *d_yhist, int H, const double *d_thist, int tcap,
+            vk_colstate *d_st, void *stream);
+/* Host copies of the history vk_integrate keeps for conv() (save_step's
+ * y_time / t_time, op.py:1087-1105): with write = 1 loads t_time[0:count]
+ * of every column (h_t: [ncols][count]) and y_time[q] for q0 <= q < count
+ * (h_y: [count - q0][L][ni]) into the integrator, e.g. to resume a run at
+ * para.count > 0; with write = 0 reads them back.  count - q0 may not exceed
+ * the integrator's history length; either host pointer may be NULL.
+ * Synchronises the device. */
+int vk_integrator_history(vk_integrator *it, int write, int count, int q0,
+                          double *h_t, double *h_y);
 /* update_mu_dz (op.py:944-984) for every column that is not done */
 int vk_update_mu_dz(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
                     const vk_geom *geom, const double *d_ymix, const vk_colstate *d_st,

@@ -240,7 +240,217 @@ def main():
     np.savez_compressed(
         os.path.join(OUT, 'ref_readrate_CHO_thermo.npz'),
         Tco=atm.Tco, M=atm.M, index=np.array(idx), eq=np.array([var2.Rf[i] for i in idx]), kf=kf)
+
+    conv_golden(cfg, op, store, make_atm, ini, solver, var, atm, para, y0, species, geom0)
     print('wrote', OUT)
+
+
+def _run_to_convergence(cfg, op, store, make_atm, ini, solver, var, atm, para, y0, geom0):
+    """Integration.__call__ from y0 (restored geometry) until stop() ends it."""
+    for key, v in geom0.items():
+        setattr(atm, key[:-1], v.copy())
+    var.y = y0.copy()
+    var.ymix = var.y / np.vstack(np.sum(var.y, axis=1))
+    var = ini.ele_sum(var)
+    var.dt = cfg.dttry
+    var.t = 0
+    var.longdy, var.longdydt = 1., 1.
+    var.y_time, var.t_time, var.atom_loss_time = [], [], []
+    var.atom_loss = {a: 0. for a in var.atom_loss}
+    para.count = 0
+    para.delta_count = para.nega_count = para.loss_count = 0
+    para.end_case = 0
+    integ = op.Integration(solver, op.Output())
+    with contextlib.redirect_stdout(io.StringIO()):
+        integ(var, atm, para, make_atm)
+    return var, para, integ
+
+
+def conv_golden(cfg, op, store, make_atm, ini, solver, var, atm, para, y0, species, geom0):
+    """Reference runs that end in steady state (end_case 1), and conv() on
+    crafted histories that reach each branch of its decision (op.py:1018-1085)."""
+    gold = {}
+    cfg.update_frq = 100
+    cfg.count_min = 120
+    cfg.trun_min = 1e2
+    cfg.conv_step = 500
+    cfg.count_max = 4000
+    runs = {}
+    # run a: nothing ignored.  Its conv() tests are recorded (longdy/ymix per
+    # entry, longdy and the time span), and run b ignores the smallest set of
+    # species (largest longdy/ymix first, at most three) that would have let
+    # an earlier test of run a pass; conver_ignore changes nothing else, so
+    # run b follows run a's trajectory and stops at that earlier step.
+    conv_impl = op.Integration.conv
+    record_a = None
+    for tag in ('a', 'b'):
+        record = []
+
+        def conv_rec(self, var_, para_, atm_, *a, **kw):
+            snap = dict(count=para_.count, t=var_.t, ymix=var_.ymix.copy(), y_time=var_.y_time,
+                        t_time=np.array(var_.t_time), Hp=atm_.Hp.copy())
+            res = conv_impl(self, var_, para_, atm_, *a, **kw)
+            snap.update(res=res, longdy=var_.longdy, longdydt=var_.longdydt,
+                        wv=np.nan_to_num(np.asarray(para_.where_varies_most), nan=0.0, posinf=0.0).copy(),
+                        span=var_.longdy / var_.longdydt if var_.longdydt > 0 else np.inf)
+            record.append(snap)
+            return res
+        if tag == 'a':
+            cfg.conver_ignore = []
+        else:
+            slope_min = max(min(np.amin(atm.Kzz / (0.1 * atm.Hp[:-1]) ** 2), 1.e-8), 1.e-10)
+            pick = None
+            for sn in record_a:
+                cnt, t, wv, span, res = sn['count'], sn['t'], sn['wv'], sn['span'], sn['res']
+                if res or not (t > cfg.trun_min and cnt > cfg.count_min):
+                    continue
+                per_sp = wv.max(axis=0)
+                order = np.argsort(-per_sp)
+                for nign in (1, 2, 3):
+                    rest = np.delete(per_sp, order[:nign])
+                    ld = rest.max() if rest.size else 0.0
+                    if (ld < cfg.yconv_cri and ld / span < cfg.slope_cri or
+                            ld < cfg.yconv_min and ld / span < slope_min):
+                        pick = [species[int(q)] for q in order[:nign]]
+                        break
+                if pick:
+                    break
+            assert pick, 'no earlier test of run a passes with up to three species ignored'
+            cfg.conver_ignore = pick
+        op.Integration.conv = conv_rec
+        var, para, integ = _run_to_convergence(cfg, op, store, make_atm, ini, solver, var, atm, para, y0, geom0)
+        op.Integration.conv = conv_impl
+        if tag == 'a':
+            record_a = record
+        count = para.count
+        keep = sorted(set([100, 200, 300, count // 2, count - 1]) & set(range(count)))
+        runs[tag] = count
+        gold['run%s_ignore' % tag] = np.array(cfg.conver_ignore, dtype='U16')
+        gold['run%s_final' % tag] = np.array([var.dt, para.delta, var.longdy, var.longdydt, para.end_case,
+                                               para.count, para.delta_count, para.nega_count, para.loss_count,
+                                               var.t])
+        gold['run%s_t' % tag] = np.array(var.t_time)
+        gold['run%s_keep' % tag] = np.array(keep)
+        gold['run%s_y' % tag] = np.stack([var.y_time[q] for q in keep])
+        gold['run%s_ymix' % tag] = var.ymix.copy()
+        gold['run%s_geom' % tag] = np.stack([atm.g, atm.Hp])
+        # the last three conv() tests with the states they read: y_time[count-1]
+        # and the states around the window start (argmin |t_time - t/2|,
+        # stepped back from count-1, cut at count - conv_step)
+        for j, sn in enumerate(record[-3:]):
+            cnt, tt_ = sn['count'], sn['t_time']
+            ix = int(np.abs(tt_ - sn['t'] * cfg.st_factor).argmin())
+            if ix == cnt - 1:
+                ix -= 1
+            ix = max(cnt - cfg.conv_step, ix)
+            qs = sorted(set(q for q in (ix - 1, ix, ix + 1, cnt - 1) if 0 <= q < cnt))
+            pre = 'run%s_ct%d_' % (tag, j)
+            gold[pre + 'meta'] = np.array([cnt, sn['t'], float(sn['res']), sn['longdy'], sn['longdydt']])
+            gold[pre + 'ymix'] = sn['ymix']
+            gold[pre + 'q'] = np.array(qs)
+            gold[pre + 'ys'] = np.stack([sn['y_time'][q] for q in qs])
+            gold[pre + 't_time'] = tt_[:cnt]
+            gold[pre + 'Hp'] = sn['Hp']
+    gold['run_cfg'] = np.array([cfg.update_frq, cfg.count_min, cfg.trun_min, cfg.conv_step, cfg.count_max])
+
+    # ---- conv() on crafted histories -----------------------------------------------
+    # nz = 4 layers, ni = 6 species (the first six of the network, so that
+    # conver_ignore names resolve through species.index)
+    rng = np.random.default_rng(77)
+    nzc, nic = 4, 6
+    catm = store.AtmData()
+    catm.n_0 = 10.0 ** rng.uniform(14, 18, nzc)
+    catm.Kzz = np.array([450., 1e3, 1e4])   # slope_min = 5e-9, inside its clip range
+    catm.Hp = np.array([3e6, 3.2e6, 3.5e6, 3.9e6])
+    integ = op.Integration(solver, op.Output())
+    base = 10.0 ** rng.uniform(-8, -1, (nzc, nic))
+    base /= base.sum(axis=1)[:, None]
+    cases = []
+
+    def case(name, count, t_time, drift, conv_step=500, ignore=(), aflux=0.0, tiny=None):
+        """y_time[q] = n_0 * base * (1 + drift[q]); tiny marks entries set below atol."""
+        yt = []
+        for q in range(count):
+            ym = base * (1.0 + drift[q])
+            y = np.vstack(catm.n_0) * ym
+            if tiny is not None:
+                y[tiny] = 0.01 * (1.0 + drift[q][tiny])
+            yt.append(y)
+        cases.append(dict(name=name, count=count, t_time=np.asarray(t_time, float), y_time=np.stack(yt),
+                          conv_step=conv_step, ignore=list(ignore), aflux=aflux))
+
+    n = 60
+    tt = np.cumsum(np.geomspace(1.0, 1e10, n))      # geometric steps: argmin near the end
+    tlin = np.arange(1.0, n + 1.0)                   # uniform steps: argmin mid-history
+
+    def jump(a):
+        """no change until the last state, which moves every entry by a factor 1 + a"""
+        d = np.zeros((n, nzc, nic))
+        d[-1] = a
+        return d
+
+    ramp = lambda a: a * np.linspace(0.0, 1.0, n)[:, None, None] * np.ones((n, nzc, nic))
+    # branch 1: longdy < yconv_cri and longdydt < slope_cri
+    case('branch1', n, tt, jump(1e-3))
+    # branch 2: yconv_cri <= longdy < yconv_min and longdydt < slope_min
+    case('branch2', n, tt * 1e8, jump(0.05))
+    # longdy between the two bounds but longdydt between slope_min and slope_cri
+    case('between_slopes', n, tt * 1e-5, jump(0.05))
+    # neither: longdy small but changing too fast
+    case('too_fast', n, np.cumsum(np.geomspace(1e-6, 1.0, n)), jump(5e-3))
+    # neither: longdy above yconv_min
+    case('too_large', n, tt * 1e8, jump(0.5))
+    # branch 1 met but the actinic-flux change is too large
+    case('aflux', n, tt, jump(1e-3), aflux=0.5)
+    # one species moves: not converged, converged once it is ignored
+    d1 = jump(1e-3)
+    d1[-1, :, 2] = 0.8
+    case('one_species', n, tt, d1)
+    case('one_species_ignored', n, tt, d1, ignore=(species[2],))
+    # entries below atol are masked out whatever they do
+    d2 = jump(1e-3)
+    d2[-1, 1, 4] = 5.0
+    mask = np.zeros((nzc, nic), bool)
+    mask[1, 4] = True
+    case('below_atol', n, tt, d2, tiny=mask)
+    # the last step dominates t: argmin lands on count-1 and steps back one
+    tl = np.append(np.arange(1.0, n), 1e12)
+    case('last_step_dominates', n, tl, ramp(1e-3))
+    # the window is cut at count - conv_step (argmin mid-history)
+    case('window_cut', n, tlin, ramp(2e-3), conv_step=10)
+    case('window_full', n, tlin, ramp(2e-3))
+    out = []
+    for c in cases:
+        cv = store.Variables()
+        cp = store.Parameters()
+        cv.y_time = list(c['y_time'])
+        cv.t_time = list(c['t_time'])
+        cv.y = c['y_time'][-1]
+        cv.ymix = cv.y / np.vstack(np.sum(cv.y, axis=1))
+        cv.t = c['t_time'][-1]
+        cv.aflux_change = c['aflux']
+        cp.count = c['count']
+        cfg.conv_step = c['conv_step']
+        cfg.conver_ignore = c['ignore']
+        with contextlib.redirect_stdout(io.StringIO()):
+            res = integ.conv(cv, cp, catm)
+        out.append((c, bool(res), cv.longdy, cv.longdydt))
+    for q, (c, res, ld, ldt) in enumerate(out):
+        pre = 'case%d_' % q
+        gold[pre + 'name'] = np.array(c['name'])
+        gold[pre + 'y_time'] = c['y_time']
+        gold[pre + 't_time'] = c['t_time']
+        gold[pre + 'meta'] = np.array([c['count'], c['conv_step'], c['aflux']])
+        gold[pre + 'ignore'] = np.array(c['ignore'], dtype='U16')
+        gold[pre + 'out'] = np.array([float(res), ld, ldt])
+    gold['ncases'] = np.array(len(out))
+    gold['case_n0'] = catm.n_0
+    gold['case_Kzz'] = catm.Kzz
+    gold['case_Hp'] = catm.Hp
+    gold['case_species'] = np.array(species[:nic])
+    np.savez_compressed(os.path.join(OUT, 'ref_conv_nz8.npz'), **gold)
+    print('conv runs: a %d steps, b %d steps (ignore %s); cases %s' % (
+        runs['a'], runs['b'], list(gold['runb_ignore']), [(c['name'], r) for c, r, _, _ in out]))
 
 
 if __name__ == '__main__':

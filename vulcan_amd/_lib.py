@@ -39,12 +39,12 @@ class Params(C.Structure):
         'nega_cut', 'loss_eps', 'trun_min', 'runtime', 'st_factor', 'yconv_cri', 'slope_cri', 'yconv_min',
         'flux_cri')] + [(n, C.c_int) for n in (
             'count_min', 'count_max', 'conv_step', 'update_frq', 'use_moldiff', 'use_conv', 'use_update_mu',
-            'refine')]
+            'refine')] + [('conv_ignore', C.c_uint * 5)]
 
 
 COLSTATE_DTYPE = np.dtype([
     ('t', 'f8'), ('dt', 'f8'), ('delta', 'f8'), ('longdy', 'f8'), ('longdydt', 'f8'), ('small_y', 'f8'),
-    ('nega_y', 'f8'), ('pad0', 'f8'), ('count', 'i4'), ('attempts', 'i4'), ('nega_count', 'i4'),
+    ('nega_y', 'f8'), ('aflux_change', 'f8'), ('count', 'i4'), ('attempts', 'i4'), ('nega_count', 'i4'),
     ('loss_count', 'i4'), ('delta_count', 'i4'), ('end_case', 'i4'), ('retry', 'i4'), ('done', 'i4')])
 assert COLSTATE_DTYPE.itemsize == 96
 
@@ -76,6 +76,8 @@ _SIGS = {
     'vk_ros2_solver': [VP, C.POINTER(Atm), C.POINTER(Params), VP, VP, VP, VP, VP],
     'vk_integrate': [VP, C.POINTER(Atm), C.POINTER(Geom), C.POINTER(Params), VP, VP, VP, VP, VP, VP, VP, VP,
                      C.c_int, VP],
+    'vk_conv': [VP, VP, C.POINTER(Atm), C.POINTER(Params), VP, VP, VP, C.c_int, VP, C.c_int, VP, VP],
+    'vk_integrator_history': [VP, C.c_int, C.c_int, C.c_int, VP, VP],
     'vk_update_mu_dz': [VP, VP, C.POINTER(Atm), C.POINTER(Geom), VP, VP, C.c_int, VP],
     'vk_atom_sums': [VP, VP, VP, VP, VP],
     'vk_integrator_set_timing': [VP, C.c_int],

@@ -19,7 +19,10 @@ r_jup = 7.1492E9
 
 @dataclasses.dataclass
 class Config:
-    ode_solver: str = 'Ros2'          # vulcan_cfg.py:128
+    # vulcan_cfg.py:128; the reference dispatches on the name
+    # (getattr(op, ode_solver)), so a registered device solver may carry any
+    # name (for example 'Ros2_amd'); the step is Ros2 whatever it says
+    ode_solver: str = 'Ros2'
     use_moldiff: bool = True          # vulcan_cfg.py:76
     use_vm_mol: bool = False          # vulcan_cfg.py:77
     use_vz: bool = False              # vulcan_cfg.py:78
@@ -72,5 +75,3 @@ class Config:
             raise NotImplementedError(
                 'configuration switches not supported by the device Ros2 path: '
                 + ', '.join(bad))
-        if self.ode_solver != 'Ros2':
-            raise NotImplementedError('only ode_solver = "Ros2" is provided')

@@ -4,7 +4,7 @@
 #include <vector>
 #include "../../include/vulcan_amd.h"
 
-#define VK_NI_MAX 160        // largest network the kernels are compiled for
+#define VK_NI_MAX VK_NI_LIMIT  // largest network the kernels are compiled for
 #define VK_NATOM_MAX 8
 #define VK_JAC_LIGHT 8   // term lists up to this length are evaluated in the dense pass
 

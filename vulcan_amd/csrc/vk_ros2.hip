@@ -400,10 +400,12 @@ __global__ void __launch_bounds__(256) k_conv(int ni, const int *__restrict__ of
         for (int q = threadIdx.x; q < nz * ni; q += blockDim.x) {
             const int j = q / ni;
             const size_t e = (size_t)(o + j) * ni + (q - j * ni);
+            const int i = q - j * ni;
             double d = fabs((ycur[e] - yold[e]) / atm.M[o + j]);
             const double mx = ymix[e];
             if (mx < prm.mtol_conv) d = 0;
             if (ycur[e] < prm.atol) d = 0;
+            if ((prm.conv_ignore[i >> 5] >> (i & 31)) & 1u) d = 0;   // conver_ignore, op.py:1045-1046
             if (mx > 0) lmax = fmax(lmax, d / mx);
         }
         __syncthreads();
@@ -426,7 +428,7 @@ __global__ void __launch_bounds__(256) k_conv(int ni, const int *__restrict__ of
             s.longdy = longdy;
             s.longdydt = longdydt;
             conv = ((longdy < prm.yconv_cri && longdydt < prm.slope_cri) ||
-                    (longdy < prm.yconv_min && longdydt < smin)) && 0.0 < prm.flux_cri;
+                    (longdy < prm.yconv_min && longdydt < smin)) && s.aflux_change < prm.flux_cri;
         }
     }
     if (threadIdx.x == 0) {
@@ -878,6 +880,45 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     for (int a = 0; a < n_attempts; a++) VK_CHECK(hipGraphLaunch(it->gexec, it->gstream));
     VK_CHECK(hipEventRecord(it->ev1, it->gstream));
     VK_CHECK(hipStreamWaitEvent(s, it->ev1, 0));
+    return 0;
+}
+
+extern "C" int vk_conv(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const vk_params *prm,
+                       const double *d_y, const double *d_ymix, const double *d_yhist, int H, const double *d_thist,
+                       int tcap, vk_colstate *d_st, void *stream) {
+    if (!plan || !cols || !atm || !prm || !d_y || !d_ymix || !d_st) return VK_EINVAL;
+    if (prm->use_conv && (!d_yhist || !d_thist || H < 1 || tcap < 1)) return VK_EINVAL;
+    hipLaunchKernelGGL(k_conv, dim3(cols->ncols), dim3(256), 0, (hipStream_t)stream, plan->ni, cols->d_off, *prm,
+                       *atm, d_y, d_ymix, nullptr, d_st, d_yhist, prm->use_conv ? H : 0, (size_t)cols->L * plan->ni,
+                       d_thist, tcap);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int vk_integrator_history(vk_integrator *it, int write, int count, int q0, double *h_t, double *h_y) {
+    if (!it || it->history <= 0 || count < 0 || q0 < 0 || q0 > count || count - q0 > it->history) return VK_EINVAL;
+    VK_CHECK(hipDeviceSynchronize());
+    if (write) {
+        int r = ensure_thist(it, count);
+        if (r) return r;
+    } else if (count > it->tcap) {
+        return VK_EINVAL;
+    }
+    const hipMemcpyKind dir = write ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    const int nc = it->cols->ncols;
+    const size_t Lni = (size_t)it->cols->L * it->plan->ni;
+    if (h_t && count > 0) {
+        for (int c = 0; c < nc; c++) {
+            double *dv = it->thist + (size_t)c * it->tcap, *hv = h_t + (size_t)c * count;
+            VK_CHECK(write ? hipMemcpy(dv, hv, sizeof(double) * count, dir) : hipMemcpy(hv, dv, sizeof(double) * count, dir));
+        }
+    }
+    if (h_y) {
+        for (int q = q0; q < count; q++) {
+            double *dv = it->yhist + (size_t)(q % it->history) * Lni, *hv = h_y + (size_t)(q - q0) * Lni;
+            VK_CHECK(write ? hipMemcpy(dv, hv, sizeof(double) * Lni, dir) : hipMemcpy(hv, dv, sizeof(double) * Lni, dir));
+        }
+    }
     return 0;
 }
 

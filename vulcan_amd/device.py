@@ -126,7 +126,19 @@ class DeviceColumns:
             pass
 
 
-def params_from(cfg: Config, use_conv: bool = True, use_update_mu: bool = True) -> _lib.Params:
+def conv_ignore_mask(cfg: Config, species: Sequence[str]) -> np.ndarray:
+    """Bit mask of ``cfg.conver_ignore`` over the network's species (bit
+    i % 32 of word i // 32); an unknown name raises ValueError, as the
+    reference's ``species.index(sp)`` does (op.py:1045-1046)."""
+    mask = np.zeros(5, np.uint32)
+    for sp in cfg.conver_ignore:
+        i = list(species).index(sp)
+        mask[i // 32] |= np.uint32(1 << (i % 32))
+    return mask
+
+
+def params_from(cfg: Config, use_conv: bool = True, use_update_mu: bool = True,
+                species: Optional[Sequence[str]] = None) -> _lib.Params:
     p = _lib.Params()
     for name in ('dt_min', 'dt_max', 'dt_var_min', 'dt_var_max', 'rtol', 'atol', 'mtol', 'mtol_conv', 'pos_cut',
                  'nega_cut', 'loss_eps', 'trun_min', 'runtime', 'st_factor', 'yconv_cri', 'slope_cri', 'yconv_min',
@@ -137,6 +149,11 @@ def params_from(cfg: Config, use_conv: bool = True, use_update_mu: bool = True) 
     p.use_conv = int(use_conv)
     p.use_update_mu = int(use_update_mu)
     p.refine = int(getattr(cfg, 'refine', 1))
+    if cfg.conver_ignore:
+        if species is None:
+            raise ValueError('conver_ignore needs the species list')
+        for w, m in enumerate(conv_ignore_mask(cfg, species)):
+            p.conv_ignore[w] = int(m)
     return p
 
 
@@ -236,7 +253,7 @@ class ColumnBatch:
     def params(self, use_conv=None, use_update_mu=True):
         if use_conv is None:
             use_conv = self.history > 0
-        return params_from(self.cfg, use_conv=use_conv, use_update_mu=use_update_mu)
+        return params_from(self.cfg, use_conv=use_conv, use_update_mu=use_update_mu, species=self.plan.net.species)
 
     def attempt(self):
         """One Ros2.solver attempt on every active column (op.py:2860-3007)."""
@@ -272,6 +289,25 @@ class ColumnBatch:
         n = C.c_longlong()
         _lib.check(self._lib.vk_integrator_stats(self.it, C.byref(ms), C.byref(n)), 'vk_integrator_stats')
         return ms.value, n.value
+
+    def read_history(self, q0: int, count: int):
+        """(t_time [ncols][count], y_time[q0:count] as [count-q0][L][ni]) from
+        the integrator (the last ``history`` states are kept)."""
+        t = np.zeros((self.layout.ncols, count))
+        y = np.zeros((count - q0, self.layout.L, self.ni))
+        _lib.check(self._lib.vk_integrator_history(self.it, 0, int(count), int(q0), t.ctypes.data, y.ctypes.data),
+                   'vk_integrator_history')
+        return t, y
+
+    def write_history(self, t_time: np.ndarray, y_time: np.ndarray, q0: int):
+        """Load t_time [ncols][count] and y_time[q0:count] ([count-q0][L][ni])
+        into the integrator (resuming a run at count > 0)."""
+        t = np.ascontiguousarray(t_time, dtype=np.float64)
+        y = np.ascontiguousarray(y_time, dtype=np.float64)
+        count = t.shape[1]
+        assert t.shape[0] == self.layout.ncols and y.shape == (count - q0, self.layout.L, self.ni)
+        _lib.check(self._lib.vk_integrator_history(self.it, 1, int(count), int(q0), t.ctypes.data, y.ctypes.data),
+                   'vk_integrator_history')
 
     # -- host views ------------------------------------------------------------------
     def host_state(self) -> np.ndarray:

@@ -52,6 +52,11 @@ class Ros2:
         self.net = net
         self.cfg = cfg or Config()
         self.cfg.check_supported()
+        for sp in self.cfg.conver_ignore:
+            if sp not in net.species:
+                # the reference fails the same way, at its first conv() test
+                # (species.index(sp), op.py:1045-1046)
+                raise ValueError('conver_ignore names %r, which is not a species of the network' % (sp,))
         self.chem = ChemFuns(net)
         self.atom_list = net.atoms()
         self._atom_mat = net.atom_matrix()
@@ -191,17 +196,26 @@ class Ros2:
 class Integration:
     """Integration.__call__ (op.py:808-936) resident on the device.
 
-    ``var`` must carry ``y``, ``ymix``, ``k``, ``dt``, ``t`` and the element
-    dicts; ``atm`` the column arrays.  On return ``var.y``, ``var.ymix``,
-    ``var.t``, ``var.dt``, ``var.longdy``, ``var.longdydt``, ``var.t_time``,
-    ``var.y_time`` (the last ``conv_step`` states), ``para.count``,
-    ``para.end_case`` and the reject counters are set.
+    ``var`` must carry ``y``, ``ymix``, ``k``, ``dt``, ``t``, ``atom_ini`` and
+    ``atom_loss``; ``atm`` the column arrays.  A run resumed at
+    ``para.count > 0`` needs ``var.t_time`` (``count`` entries) and
+    ``var.y_time`` (at least the last ``conv_step + 1`` states), which are
+    loaded into the device history that ``conv()`` reads.  On return
+    ``var.y``, ``var.ymix``, ``var.t``, ``var.dt``, ``var.longdy``,
+    ``var.longdydt``, ``var.atom_loss``, ``var.t_time`` (every step),
+    ``var.y_time`` (every step with ``keep_y_time``, else the last
+    ``conv_step + 1``), ``para.count``, ``para.end_case``, ``para.delta`` and
+    the reject counters are set, and ``atm.g``/``Hp``/``dzi``/``Hpi`` hold
+    the last ``update_mu_dz`` geometry.  The host polls the device every
+    ``poll`` attempts (at most ``conv_step + 1``, the device ring's length,
+    so that every state can be copied out before it is overwritten).
     """
 
-    def __init__(self, odesolver: Ros2, output=None, poll: int = 64):
+    def __init__(self, odesolver: Ros2, output=None, poll: int = 64, keep_y_time: bool = True):
         self.odesolver = odesolver
         self.output = output
         self.poll = poll
+        self.keep_y_time = keep_y_time
 
     def __call__(self, var, atm, para, make_atm=None):
         import torch
@@ -217,14 +231,38 @@ class Integration:
         b.ymix.copy_(torch.as_tensor(np.ascontiguousarray(var.ymix, dtype=np.float64)))
         ai = np.array([var.atom_ini[a] for a in solver.atom_list])
         b.atom_ini.copy_(torch.as_tensor(ai[None]))
+        loss0 = getattr(var, 'atom_loss', None) or {}
+        al = np.array([float(loss0.get(a, 0.0)) for a in solver.atom_list])
+        b.atom_loss.copy_(torch.as_tensor(al[None]))
+        b.atom_loss_prev.copy_(torch.as_tensor(al[None]))   # backup() at the loop's start
+        count0 = int(getattr(para, 'count', 0) or 0)
+        t_hist = [float(t) for t in getattr(var, 't_time', [])][:count0]
+        y_hist = list(getattr(var, 'y_time', []))[:count0]
+        if count0 > 0:
+            if len(t_hist) != count0 or len(y_hist) < min(count0, H):
+                raise ValueError('resuming at para.count = %d needs var.t_time (%d entries) and the last %d '
+                                 'states of var.y_time' % (count0, count0, min(count0, H)))
+            q0 = count0 - min(count0, H)
+            b.write_history(np.array(t_hist)[None], np.stack(y_hist[len(y_hist) - (count0 - q0):]), q0)
         st = b.host_state()
-        st['dt'], st['t'], st['count'] = var.dt, var.t, getattr(para, 'count', 0)
+        st['dt'], st['t'], st['count'] = var.dt, var.t, count0
         b.set_host_state(st)
+        poll = max(1, min(self.poll, H))
+        seen = count0
+        y_time = y_hist if self.keep_y_time else []
         while True:
-            b.integrate(self.poll)
+            b.integrate(poll)
             h = b.host_state()[0]
+            cnt = int(h['count'])
+            if self.keep_y_time and cnt > seen:
+                _, ys = b.read_history(seen, cnt)
+                y_time.extend(ys[q] for q in range(cnt - seen))
+            seen = cnt
             if h['done']:
                 break
+        t_all, ys = b.read_history(max(0, cnt - H) if not self.keep_y_time else cnt, cnt)
+        var.t_time = [float(t) for t in t_all[0]]
+        var.y_time = y_time if self.keep_y_time else [ys[q] for q in range(ys.shape[0])]
         var.y, var.ymix = b.y.cpu().numpy(), b.ymix.cpu().numpy()
         var.t, var.dt = float(h['t']), float(h['dt'])
         var.longdy, var.longdydt = float(h['longdy']), float(h['longdydt'])
@@ -236,6 +274,9 @@ class Integration:
         for name in ('dzi', 'Hpi'):
             setattr(atm, name, b.column(name, 0)[:nz - 1])
         loss = b.atom_loss.cpu().numpy()[0]
+        if getattr(var, 'atom_loss', None) is None:
+            var.atom_loss = {}
         for a, atom in enumerate(solver.atom_list):
             var.atom_loss[atom] = float(loss[a])
+        b.close()
         return var
