@@ -410,6 +410,7 @@ class State:
     k1: Optional[np.ndarray] = None
     k2: Optional[np.ndarray] = None
     attempts: int = 0
+    exact_berr: float = 0.0     # backward error of the refined solves (exact=True)
 
 
 def atom_sums(y, atom_mat):
@@ -431,23 +432,47 @@ def new_state(net, y, k, cfg: Config) -> State:
     return st
 
 
+def _residual_ld(Dl, Ll, Ul, bl, x):
+    """b - A x of the block-tridiagonal system, in extended precision."""
+    r = bl - np.einsum('lij,lj->li', Dl, x)
+    r[1:] -= Ll[1:] * x[:-1]
+    r[:-1] -= Ul[:-1] * x[1:]
+    return r
+
+
+def backward_error(D, L, U, rhs, x) -> float:
+    """Componentwise backward error max_i |b - A x|_i / (|A| |x| + |b|)_i of
+    x for the block-tridiagonal system (D, L, U), in extended precision."""
+    nz, ni = D.shape[:2]
+    Dl, Ll, Ul = D.astype(np.longdouble), L.astype(np.longdouble), U.astype(np.longdouble)
+    bl = np.asarray(rhs).reshape(nz, ni).astype(np.longdouble)
+    xl = np.asarray(x).reshape(nz, ni).astype(np.longdouble)
+    r = np.abs(_residual_ld(Dl, Ll, Ul, bl, xl))
+    ax = np.einsum('lij,lj->li', np.abs(Dl), np.abs(xl)) + np.abs(bl)
+    ax[1:] += np.abs(Ll[1:] * xl[:-1])
+    ax[:-1] += np.abs(Ul[:-1] * xl[1:])
+    with np.errstate(invalid='ignore', divide='ignore'):
+        q = np.where(ax > 0, r / ax, 0.0)
+    return float(np.max(q))
+
+
 def solve_refined(D, L, U, lhs_b, bw, rhs, iters=3):
     """The banded LAPACK solve followed by `iters` steps of iterative
     refinement with the residual b - A x formed blockwise in extended
     precision: the linear solve to (nearly) full FP64 accuracy.  Not part of
     the reference; it gives the exact Ros2 step against which the
-    reference's own banded solve and the device solve are both measured."""
+    reference's own banded solve and the device solve are both measured.
+    Returns (x, componentwise backward error of the returned FP64 x)."""
     nz, ni = D.shape[:2]
     Dl, Ll, Ul = D.astype(np.longdouble), L.astype(np.longdouble), U.astype(np.longdouble)
     bl = rhs.reshape(nz, ni).astype(np.longdouble)
     x = scipy.linalg.solve_banded((bw, bw), lhs_b, rhs).reshape(nz, ni).astype(np.longdouble)
     for _ in range(iters):
-        r = bl - np.einsum('lij,lj->li', Dl, x)
-        r[1:] -= Ll[1:] * x[:-1]
-        r[:-1] -= Ul[:-1] * x[1:]
+        r = _residual_ld(Dl, Ll, Ul, bl, x)
         dx = scipy.linalg.solve_banded((bw, bw), lhs_b, r.astype(np.float64).ravel())
         x = x + dx.reshape(nz, ni).astype(np.longdouble)
-    return x.astype(np.float64).ravel()
+    x64 = x.astype(np.float64).ravel()
+    return x64, backward_error(D, L, U, rhs, x64)
 
 
 def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False, exact=False):
@@ -478,8 +503,14 @@ def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False, 
         del lhs
     else:
         lhs_b, bw = band_from_blocks(D, L, U)
-    solve = ((lambda rhs_: solve_refined(D, L, U, lhs_b, bw, rhs_)) if exact
-             else (lambda rhs_: scipy.linalg.solve_banded((bw, bw), lhs_b, rhs_)))
+    st.exact_berr = 0.0
+
+    def solve(rhs_):
+        if not exact:
+            return scipy.linalg.solve_banded((bw, bw), lhs_b, rhs_)
+        x_, berr = solve_refined(D, L, U, lhs_b, bw, rhs_)
+        st.exact_berr = max(st.exact_berr, berr)
+        return x_
     k1_flat = solve(df)
     k1 = k1_flat.reshape(y.shape)
     yk2 = y + k1 / r

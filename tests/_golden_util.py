@@ -46,3 +46,15 @@ def rel_err(a, b, floor=0.0):
     with np.errstate(divide='ignore', invalid='ignore'):
         r = np.where(den > 0, np.abs(a - b) / den, np.abs(a - b))
     return float(np.max(r)) if r.size else 0.0
+
+
+def network_fixture(name) -> nw.Network:
+    """A real VULCAN network's device tables (tests/golden/make_network_tables.py)."""
+    d = load(name)
+    net = nw.Network(species=[str(s) for s in d['species']], elements=[str(e) for e in d['elements']],
+                     compo=d['compo'], mass=d['mass'], nasa9=d['nasa9'], reac=d['reac'], prod=d['prod'],
+                     kind=d['kind'], par=d['par'].copy(), rev=d['rev'], labels=[str(s) for s in d['labels']],
+                     atom_list=[str(a) for a in d['atom_list']])
+    net.validate()
+    net.vulcan_index = d['vulcan_index']
+    return net

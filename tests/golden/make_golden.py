@@ -17,7 +17,10 @@ are off) and records, for the network of the shipped generated
 * one ``Ros2.solver`` attempt, one ``one_step`` + ``step_size``,
 * a short ``Integration`` trajectory with rejections, ``update_mu_dz`` calls
   and the ``conv`` test,
-* ``ReadRate.read_rate`` forward rates of ``thermo/CHO_thermo_network.txt``.
+* ``ReadRate.read_rate`` forward rates of ``thermo/CHO_thermo_network.txt``
+  and ``thermo/SNCHO_photo_network_2024.txt``,
+* two ``Integration`` runs to steady state and ``conv()`` on crafted
+  histories (``ref_conv_nz8.npz``).
 
 Inputs are seeded synthetic columns; outputs are what the reference computes
 on them.  Only data (arrays, strings) is written, as compressed ``.npz``.
@@ -240,6 +243,24 @@ def main():
     np.savez_compressed(
         os.path.join(OUT, 'ref_readrate_CHO_thermo.npz'),
         Tco=atm.Tco, M=atm.M, index=np.array(idx), eq=np.array([var2.Rf[i] for i in idx]), kf=kf)
+
+    # the same for the 2024 S-N-C-H-O photochemical network (ReadRate reads
+    # whichever file vulcan_cfg.network names); a wider temperature range
+    # reaches both NASA9 ranges and the falloff region of its three-body rows
+    atm3 = store.AtmData()
+    atm3.Tco = np.linspace(2500., 300., NZ)
+    atm3.M = 10.0 ** np.linspace(20., 12., NZ)
+    cfg.network = 'thermo/SNCHO_photo_network_2024.txt'
+    var3 = store.Variables()
+    rr3 = op.ReadRate()
+    with contextlib.redirect_stdout(io.StringIO()):
+        var3 = rr3.read_rate(var3, atm3)
+    idx3 = sorted(i for i in var3.Rf if i % 2 == 1)
+    np.savez_compressed(
+        os.path.join(OUT, 'ref_readrate_SNCHO_2024.npz'),
+        Tco=atm3.Tco, M=atm3.M, index=np.array(idx3), eq=np.array([var3.Rf[i] for i in idx3]),
+        kf=np.stack([var3.k[i] for i in idx3], axis=1))
+    cfg.network = 'thermo/CHO_thermo_network.txt'
 
     conv_golden(cfg, op, store, make_atm, ini, solver, var, atm, para, y0, species, geom0)
     print('wrote', OUT)

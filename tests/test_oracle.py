@@ -422,8 +422,11 @@ def test_solve_refined_is_the_accurate_solve():
     U = rng.standard_normal((nz, ni)); U[-1] = 0
     b = rng.standard_normal(nz * ni)
     ab, bw = O.band_from_blocks(D, L, U)
-    x = O.solve_refined(D, L, U, ab, bw, b)
+    x, berr = O.solve_refined(D, L, U, ab, bw, b)
     assert np.allclose(x, np.linalg.solve(O.lhs_dense(D, L, U), b), rtol=1e-14, atol=1e-14)
+    assert berr < 1e-15
+    # the backward error notices a wrong solution
+    assert O.backward_error(D, L, U, b, x * (1 + 1e-9)) > 1e-11
     c = W.small_case()
     k = O.rates(c.net, c.col.Tco, c.col.M, c.col.tau)
     y = c.y0
@@ -433,4 +436,8 @@ def test_solve_refined_is_the_accurate_solve():
     ab, bw = O.band_from_blocks(D, L, U)
     A = O.lhs_dense(D, L, U).astype(np.longdouble)
     res = lambda v: np.max(np.abs(df.astype(np.longdouble) - A @ v.astype(np.longdouble)))
-    assert res(O.solve_refined(D, L, U, ab, bw, df)) <= res(scipy.linalg.solve_banded((bw, bw), ab, df))
+    xr, berr = O.solve_refined(D, L, U, ab, bw, df)
+    xb = scipy.linalg.solve_banded((bw, bw), ab, df)
+    assert res(xr) <= res(xb)
+    # refinement converged: backward error at FP64 rounding, below the banded solve's
+    assert berr < 1e-15 and berr <= O.backward_error(D, L, U, df, xb)

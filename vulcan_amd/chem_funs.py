@@ -77,7 +77,7 @@ class ChemFuns:
         plan, cols = self._device(nz)
         dev = torch.device('cuda')
         yd = torch.as_tensor(y, device=dev)
-        Md = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(M, (nz,)), dtype=np.float64), device=dev)
+        Md = torch.as_tensor(np.array(np.broadcast_to(M, (nz,)), dtype=np.float64), device=dev)
         kd = torch.as_tensor(k_dict_to_array(k, nz, self.nr), device=dev)
         out = torch.empty(out_shape, dtype=torch.float64, device=dev)
         lib = _lib.load()

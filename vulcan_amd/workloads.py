@@ -40,8 +40,11 @@ SOLAR = {'H': 1.0, 'O': 5.37E-4, 'C': 2.95E-4, 'N': 7.08E-5, 'He': 0.0838, 'S': 
 
 
 def element_vector(net: Network, abund=None) -> np.ndarray:
+    """Element abundances relative to H; zero for elements no species of the
+    network carries (their conservation row would be empty)."""
     abund = dict(SOLAR, **(abund or {}))
-    return np.array([abund.get(e, 0.0) for e in net.elements])
+    present = np.asarray(net.compo).any(axis=0)
+    return np.array([abund.get(e, 0.0) if present[q] else 0.0 for q, e in enumerate(net.elements)])
 
 SEED_SMALL, SEED_FULL, SEED_BATCH, SEED_MIXED = 1001, 1002, 1003, 1004
 
