@@ -204,6 +204,22 @@ int vk_tune_factor(vk_plan *plan, const vk_columns *cols, int reps, float *ms, v
 int vk_block_solve_system(const vk_plan *plan, const vk_columns *cols, const double *d_D,
                           const double *d_Lsub, const double *d_Usup, const double *d_b, double *d_x,
                           int refine, int *d_info, void *stream);
+/* vk_block_solve_system that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace (inverse blocks, refinement vectors and, for
+ * cyclic reduction, its coupling and operator blocks) comes from one static
+ * device array of VK_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; a system that needs more returns VK_ETOOBIG (for cyclic reduction
+ * about one nz = 150, ni = 93 column; for the chain variants about 1400
+ * layers at ni = 93).  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other streams, other host threads) race; calls
+ * ordered on one stream do not.  Nothing in the scratch needs to start at
+ * zero.  Results are those of vk_block_solve_system. */
+#define VK_NOALLOC_SCRATCH_BYTES (96u << 20)
+int vk_block_solve_system_noalloc(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                                  const double *d_Lsub, const double *d_Usup, const double *d_b,
+                                  double *d_x, int refine, int *d_info, void *stream);
 /* Self-test of the FP64 MFMA operand layout the blocked inversion relies
  * on: out[i*16+j] = sum_k A[i][k] B[k][j] with A[i][k] = i + k/4 + 1,
  * B[k][j] = 1 - k/2 + j/8 (16 x 4 x 16), computed by one MFMA. */

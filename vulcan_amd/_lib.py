@@ -67,6 +67,7 @@ _SIGS = {
     'vk_plan_factor_variant': [VP],
     'vk_tune_factor': [VP, VP, C.c_int, C.POINTER(C.c_float), VP],
     'vk_block_solve_system': [VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP],
+    'vk_block_solve_system_noalloc': [VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP],
     'vk_selftest_mfma_f64': [VP, VP],
     'vk_selftest_wave_max_u32': [VP, VP, VP],
     'vk_block_solve': [VP, VP, VP, VP, VP, VP, VP],

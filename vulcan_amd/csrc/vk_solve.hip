@@ -25,6 +25,8 @@
 // ill-conditioned blocks (early steps from a far-from-equilibrium state);
 // one refinement step with a residual computed in double-double arithmetic
 // (k_residual) brings the solution to full FP64 accuracy.
+#include <atomic>
+
 #include "vk_blk.h"
 
 namespace {
@@ -453,6 +455,64 @@ extern "C" int vk_block_solve_system(const vk_plan *plan, const vk_columns *cols
                                     &bt.ws, s);
     for (double *p : {Dinv, work, tmp})
         if (p) (void)hipFreeAsync(p, s);
+    return r;
+}
+
+// Workspace of vk_block_solve_system_noalloc: one static array per device
+// (each device holds its own copy until the process ends).  Host code never
+// uses the array's name as an address: scratch_base() resolves it once per
+// device with hipGetSymbolAddress, and only that pointer reaches kernels.
+static __device__ __align__(256) double scratch[VK_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static int scratch_base(double **out) {
+    static std::atomic<double *> base[64];   // per device, zero-initialised
+    int dev = 0;
+    VK_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return VK_EINVAL;
+    double *p = base[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void *q = nullptr;
+        VK_CHECK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        if (!q) return VK_ENOMEM;
+        p = static_cast<double *>(q);
+        base[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int vk_block_solve_system_noalloc(const vk_plan *plan, const vk_columns *cols, const double *d_D,
+                                             const double *d_Lsub, const double *d_Usup, const double *d_b,
+                                             double *d_x, int refine, int *d_info, void *stream) {
+    if (!plan || !cols || !d_D || !d_Lsub || !d_Usup || !d_b || !d_x || refine < 0 || d_b == d_x) return VK_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    // carve the workspace of vk_block_solve_system out of the scratch, each
+    // piece on a 256-byte boundary (32 doubles)
+    const size_t nn = (size_t)plan->ni * plan->ni, L = (size_t)cols->L, ns = (size_t)cols->bcr.nsurv + 1;
+    const bool bcr = plan->factor_variant == VK_FACTOR_BCR;
+    auto up = [](size_t n) { return (n + 31) & ~(size_t)31; };
+    const size_t sizes[10] = {L * nn, L * plan->ni, L * plan->ni,
+                              bcr ? L * nn : 0, bcr ? L * nn : 0, bcr ? L * nn : 0, bcr ? ns * nn : 0,
+                              bcr ? ns * nn : 0, bcr ? L * nn : 0, bcr ? L * nn : 0};
+    size_t need = 0;
+    for (size_t n : sizes) need += up(n);
+    if (need > VK_NOALLOC_SCRATCH_BYTES / sizeof(double)) return VK_ETOOBIG;
+    double *base = nullptr;
+    int r = scratch_base(&base);
+    if (r) return r;
+    double *piece[10];
+    size_t at = 0;
+    for (int q = 0; q < 10; q++) {
+        piece[q] = sizes[q] ? base + at : nullptr;
+        at += up(sizes[q]);
+    }
+    vk_bcr_ws ws;
+    ws.A = piece[3]; ws.B = piece[4]; ws.C = piece[5]; ws.P = piece[6]; ws.Q = piece[7];
+    ws.XB = piece[8]; ws.XC = piece[9];
+    r = vk_launch_factor(plan, cols, d_D, d_Lsub, d_Usup, piece[0], d_info, nullptr, &ws, s);
+    if (!r)
+        r = vk_launch_solve_refined(plan, cols, d_D, piece[0], d_Lsub, d_Usup, d_b, d_x, piece[1], piece[2], refine,
+                                    nullptr, &ws, s);
     return r;
 }
 
