@@ -16,10 +16,13 @@ The JSON line also carries
   on the launch stream (eager attempts right after the timed region; the
   timed region itself replays the captured per-attempt HIP graph) and the
   HBM traffic per factorisation measured with rocprofv3 byte counters
-  (FACTOR_TRAFFIC below);
+  (read from the committed counter summary, TRAFFIC_FILE below);
 * ``cpu_baseline``: the CPU oracle (reference algorithm: dense lhs, band
   repack, two banded LAPACK solves per attempt) timed on the host in a
-  single-threaded subprocess on a bounded sample;
+  single-threaded subprocess on a bounded sample, and (``aggregate``) in P
+  concurrent single-threaded processes on P columns of the batch;
+* ``kernels``: HBM rates of the rate, right-hand-side and lhs-assembly
+  kernels (algorithmic bytes over HIP-event time);
 * ``batched``: the same network on 256 columns with nz uniform on [80, 200]
   (one captured launch sequence per attempt for all columns);
 * ``mixed``: 512 columns over three networks (ni = 40, 93, 150), one batch
@@ -40,12 +43,29 @@ VARIANTS = ['register Gauss-Jordan chain', 'blocked Gauss-Jordan chain (MFMA Sch
             'blocked Gauss-Jordan chain, two workgroups per CU', 'two-wave Gauss-Jordan chain',
             'block cyclic reduction (MFMA Schur updates)']
 FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
-# HBM bytes per factorisation of the nz = 150, ni = 93 column, by factor
-# variant: FETCH_SIZE + WRITE_SIZE summed over the factorisation kernels of
-# one eager attempt (python3 -m vulcan_amd.profile_kernels under
-# rocprofv3 --pmc, one counter per run), each counter scaled by its reading
-# for a 512 MiB device copy.  Algorithmic: 2 ni^2 x 8 B per layer = 20.8 MB.
-FACTOR_TRAFFIC = {4: 343401327}   # block cyclic reduction: 343.4 MB (16.5x; slab products re-read their operands)
+# HBM bytes per factorisation come from the committed byte-counter summary
+# (FETCH_SIZE + WRITE_SIZE of the factorisation kernels of eager attempts,
+# python3 -m vulcan_amd.profile_kernels under rocprofv3 --pmc, one counter
+# per run, each scaled by its reading for a 512 MiB device copy); the file
+# gives the command that made it.
+TRAFFIC_FILE = os.path.join('profiles', 'hbm_traffic_ros2_attempt.md')
+
+
+def factor_traffic(section):
+    """Factorisation bytes per attempt of a workload section ('one column',
+    '256 columns') of the counter summary, or None."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    part = text.split('## ' + section, 1)
+    if len(part) < 2:
+        return None
+    for line in part[1].split('\n## ')[0].splitlines():
+        if line.startswith('Factorisation traffic'):
+            return float(line.split()[2]) * 1e6
+    return None
 
 
 def _sync():
@@ -95,9 +115,57 @@ def bench_single(steps, warmup, graph=True):
     factor_ms = factor_timing(b, 20)
     nz, ni = c.col.nz, c.net.ni
     variant = b._lib.vk_plan_factor_variant(plan.handle)
+    kr = kernel_rates(b)
     return dict(time=dt, steps=steps, attempts=attempts, nz=nz, ni=ni, npairs=c.net.npairs,
                 factor_ms=factor_ms, t_model=float(st['t']), dt_model=float(st['dt']), variant=variant,
-                tuner_ms=[round(v, 3) for v in b.factor_ms])
+                tuner_ms=[round(v, 3) for v in b.factor_ms], kernels=kr)
+
+
+def kernel_rates(b, reps=20):
+    """Average time and HBM rate of the rate, right-hand-side and lhs-assembly
+    entry points on batch b's buffers (HIP events on the launch stream), with
+    the algorithmic bytes of each:
+      rates: write k (L nr 8) and read T, M, tau;
+      rhs (chemdf + diffdf): read k, y, the atmosphere, write the rhs;
+      lhs assembly (Jacobian + diffusion, c0 I - J): read k, y, write the
+      ni x ni blocks and the two coupling vectors."""
+    import torch
+    from vulcan_amd import _lib
+    from vulcan_amd.device import dptr, stream_handle
+    import ctypes as C
+    lib = b._lib
+    L, ni, nr = b.layout.L, b.ni, b.plan.net.nr
+    ncol = b.layout.ncols
+    out = torch.empty_like(b.y)
+    D = torch.empty((L, ni, ni), dtype=torch.float64, device='cuda')
+    Ls, Us = torch.empty_like(b.y), torch.empty_like(b.y)
+    dt = torch.full((ncol,), 1e-6, dtype=torch.float64, device='cuda')
+    ktmp = torch.empty_like(b.k)
+    atm_b = 8 * L * (ni + 8)            # Dzz, Kzz, dzi, Ti, Hpi, M, Tco, ...
+    calls = {
+        'rates': (lambda: lib.vk_rates(b.plan.handle, b.layout.handle, C.byref(b.atm), dptr(ktmp),
+                                       stream_handle()), 8 * L * (nr + 3)),
+        'rhs (chemdf + diffdf)': (lambda: lib.vk_rhs(b.plan.handle, b.layout.handle, C.byref(b.atm), dptr(b.y),
+                                                     dptr(b.k), 1, None, None, dptr(out), stream_handle()),
+                                  8 * L * (nr + 2 * ni) + atm_b),
+        'lhs assembly': (lambda: lib.vk_lhs(b.plan.handle, b.layout.handle, C.byref(b.atm), dptr(b.y), dptr(b.k), 1,
+                                            dptr(dt), dptr(D), dptr(Ls), dptr(Us), stream_handle()),
+                         8 * L * (ni * ni + nr + 3 * ni) + 8 * (L - ncol) * 2 * ni + atm_b),
+    }
+    res = {}
+    for name, (fn, nbytes) in calls.items():
+        _lib.check(fn(), name)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {'avg_ms': ms, 'algorithmic_bytes': nbytes, 'achieved_GBps': nbytes / (ms * 1e-3) / 1e9,
+                     'frac_of_hbm_peak': nbytes / (ms * 1e-3) / HBM_PEAK}
+    return res
 
 
 def bench_batched(attempts, warmup, ncols=256, graph=True):
@@ -118,8 +186,9 @@ def bench_batched(attempts, warmup, ncols=256, graph=True):
     c1 = b.host_state()['count'].astype(np.int64).sum()
     factor_ms = factor_timing(b, 3)
     nzs = np.array([c.col.nz for c in cases])
+    kr = kernel_rates(b)
     return dict(time=dt, col_steps=int(c1 - c0), attempts=attempts, ncols=ncols, sum_nz=int(nzs.sum()),
-                ni=cases[0].net.ni, factor_ms=factor_ms)
+                ni=cases[0].net.ni, factor_ms=factor_ms, kernels=kr)
 
 
 def bench_mixed(attempts, warmup, ncols=512):
@@ -166,6 +235,92 @@ print(json.dumps(dict(steps=steps, attempts=st.attempts - att0, time=dt)))
 '''
 
 
+CPU_COLUMN_SNIPPET = r'''
+import json, sys, time
+sys.path.insert(0, %(root)r)
+import numpy as np
+from oracle import ros2_oracle as O
+from vulcan_amd import workloads as W
+c = W.batched_case(%(index)d, %(ncols)d)
+k = O.rates(c.net, c.col.Tco, c.col.M, c.col.tau)
+st = O.new_state(c.net, c.y0, k, c.cfg)
+tb = O.nw.build_tables(c.net)
+budget = %(budget)f
+print('ready', flush=True)
+sys.stdin.readline()                   # every process starts timing together
+t0 = time.perf_counter(); steps = 0
+while time.perf_counter() - t0 < budget:
+    st.y_prev = st.y.copy(); st.atom_loss_prev = st.atom_loss.copy()
+    O.one_step(c.net, c.col, st, c.cfg, tb, dense_lhs=True)
+    st.y = np.vstack(c.col.M) * st.ymix
+    st.t += st.dt; st.count += 1
+    O.step_size(st, c.cfg)
+    steps += 1
+print(json.dumps(dict(steps=steps, time=time.perf_counter() - t0, nz=int(c.col.nz))))
+'''
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith('Model name:'):
+                return line.split(':', 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_aggregate(budget=20.0, pmax=16):
+    """P single-threaded oracle processes (OMP_NUM_THREADS=1 each, as
+    vulcan.py:52 sets), each on its own column of the 256-column batch,
+    concurrently for `budget` seconds: aggregate accepted column-steps/s.
+    P = the host cores this process may use, capped at `pmax` (the host share
+    of one GPU) and by memory (about 2 GB per process for the dense lhs of a
+    nz <= 200 column)."""
+    avail = len(os.sched_getaffinity(0))
+    try:
+        mem = os.sysconf('SC_PAGE_SIZE') * os.sysconf('SC_AVPHYS_PAGES')
+    except (ValueError, OSError):
+        mem = 64 << 30
+    P = max(1, min(avail, pmax, int(mem // (3 << 30))))
+    env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1', MKL_NUM_THREADS='1')
+    procs = [subprocess.Popen([sys.executable, '-c', CPU_COLUMN_SNIPPET % dict(root=ROOT, ncols=256, index=i,
+                                                                                budget=budget)],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                              env=env)
+             for i in range(P)]
+    for pr in procs:                     # set-up done everywhere, then all start
+        if pr.stdout.readline().strip() != 'ready':
+            sys.stderr.write(pr.stderr.read()[-2000:])
+            return None
+    for pr in procs:
+        pr.stdin.write('go\n')
+        pr.stdin.flush()
+    steps, nzs, times = 0, [], []
+    for pr in procs:
+        out, err = pr.communicate(timeout=600)
+        if pr.returncode != 0:
+            sys.stderr.write(err[-2000:])
+            return None
+        d = json.loads(out.strip().splitlines()[-1])
+        steps += d['steps']
+        nzs.append(d['nz'])
+        times.append(d['time'])
+    wall = max(times)
+    return dict(value=steps / wall, unit='column-steps/s', processes=P, cores_available=avail,
+                cpu_model=cpu_model(),
+                sample='%d concurrent single-threaded processes, one column of the 256-column batch each (nz %d..%d, '
+                       'ni=93), started together after their set-up: %d accepted steps in %.1f s'
+                       % (P, min(nzs), max(nzs), steps, wall))
+
+
 def cpu_baseline(budget=20.0):
     env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1', MKL_NUM_THREADS='1')
     code = CPU_SNIPPET % dict(root=ROOT, budget=budget)
@@ -191,6 +346,7 @@ def main():
     ap.add_argument('--mixed-attempts', type=int, default=10)
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--no-cpu-aggregate', action='store_true')
     ap.add_argument('--no-graph', action='store_true', help='eager launches instead of the captured HIP graph')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -228,10 +384,13 @@ def main():
                                + ' (chosen by the tuner; per-variant ms for factor + 2 solves: %s)' % s['tuner_ms'],
                      'bound': 'mfma',
                      'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP64_PEAK, 'traffic': FACTOR_TRAFFIC.get(s['variant']),
+                     'frac': achieved / FP64_PEAK, 'traffic': factor_traffic('one column'),
                      'traffic_unit': 'bytes per factorisation (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)',
+                     'traffic_source': TRAFFIC_FILE,
                      'algorithmic_bytes_per_factorisation': nz * 2 * ni * ni * 8,
                      'flops_per_factorisation': flops_launch, 'avg_factorisation_ms': s['factor_ms']},
+        'kernels': {'workload': 'the same column', 'unit': 'GB/s (algorithmic bytes / HIP-event time)',
+                    'hbm_peak_GBps': HBM_PEAK / 1e9, **s['kernels']},
     }
     if not args.no_batched:
         print('[bench] batched 256 columns: %d attempts' % args.batched_attempts, file=sys.stderr, flush=True)
@@ -243,7 +402,9 @@ def main():
                           'factor_avg_launch_ms': bt['factor_ms'],
                           'factor_tflops': fl / (bt['factor_ms'] * 1e-3) / 1e12,
                           'factor_frac': fl / (bt['factor_ms'] * 1e-3) / FP64_PEAK,
-                          'vs_single': (bt['col_steps'] / bt['time']) / value}
+                          'factor_traffic': factor_traffic('256 columns'),
+                          'vs_single': (bt['col_steps'] / bt['time']) / value,
+                          'kernels': bt['kernels']}
     if not args.no_mixed:
         print('[bench] mixed networks, 512 columns: %d attempts' % args.mixed_attempts, file=sys.stderr, flush=True)
         mx = bench_mixed(args.mixed_attempts, 3)
@@ -253,6 +414,11 @@ def main():
     if not args.no_cpu:
         print('[bench] cpu baseline (%.0f s budget)' % args.cpu_budget, file=sys.stderr, flush=True)
         out['cpu_baseline'] = cpu_baseline(args.cpu_budget)
+        if not args.no_cpu_aggregate:
+            print('[bench] cpu aggregate (%.0f s budget)' % args.cpu_budget, file=sys.stderr, flush=True)
+            agg = cpu_aggregate(args.cpu_budget)
+            if out['cpu_baseline'] is not None and agg is not None:
+                out['cpu_baseline']['aggregate'] = agg
     print(json.dumps(out))
 
 

@@ -140,10 +140,18 @@ def test_mixed_512_columns_one_attempt_per_network():
 
 @gpu
 def test_mlp_full_shape_against_oracle():
-    """307 -> 4 x 1024 (SiLU) -> 13 950 on a batch of 8: forward, masked MSE
-    loss and every layer's gradients (BF16 operands, FP32 accumulation)."""
+    """307 -> 4 x 1024 (SiLU) -> 13 950 on a batch of 8, layer by layer: each
+    device GEMM (+ bias, SiLU, BF16 cast) against the oracle's layer fed the
+    device's own BF16 input, and the gradients against the oracle's backward
+    pass through the device's own activations.
+
+    End to end the two differ by BF16 rounding flips: where device and oracle
+    FP32 sums (different summation orders) straddle a BF16 boundary, a hidden
+    unit differs by one BF16 ulp and every later layer carries that, so the
+    layers are compared on common inputs; the end-to-end loss must still
+    agree to 1e-4."""
     torch = _torch()
-    from vulcan_amd.mlp import MLP
+    from vulcan_amd.mlp import MLP, bf16_to_f32
     sizes = [307, 1024, 1024, 1024, 1024, 93 * 150]
     net = MLP(sizes, seed=5)
     rng = np.random.default_rng(9)
@@ -152,27 +160,41 @@ def test_mlp_full_shape_against_oracle():
     mask = (rng.random((8, sizes[-1])) > 0.1).astype(np.float32)
     Wt = [w.cpu().numpy().astype(np.float64) for w in net.W]
     bt = [x.cpu().numpy().astype(np.float64) for x in net.b]
-    loss_ref, Y_ref, gW_ref, gb_ref = MO.loss_and_grads(Wt, bt, X.astype(np.float64), T.astype(np.float64), mask)
+    loss_ref, _, _, _ = MO.loss_and_grads(Wt, bt, X.astype(np.float64), T.astype(np.float64), mask)
     loss, Y = net.loss_and_backward(torch.as_tensor(X, device='cuda'), torch.as_tensor(T, device='cuda'),
                                     torch.as_tensor(mask, device='cuda'))
-    # The hidden activations are rounded to BF16 on both sides; where the FP32
-    # sums of device and oracle (different summation orders) straddle a BF16
-    # rounding boundary, that unit differs by one BF16 ulp (2^-8 relative),
-    # which the next layers carry to the output.  Over 4 x 1024 units a few
-    # outputs see such a flip: the bar is one BF16 ulp of the output scale for
-    # the largest difference and 1e-3 of it for all but 0.1 % of the entries.
     assert abs(loss - loss_ref) <= 1e-4 * loss_ref
-
-    def check(a, ref, what):
-        err = np.abs(np.asarray(a, dtype=np.float64) - ref)
-        scale = np.abs(ref).max()
-        q = float(np.quantile(err, 0.999))
-        print('%s: max err %.2e, 99.9%% %.2e of scale %.2e' % (what, err.max(), q, scale))
-        assert err.max() <= 2.0 ** -8 * scale, what
-        assert q <= 1e-3 * scale, what
-
-    check(Y.cpu().numpy(), Y_ref, 'output')
+    acts_t, Zs_t = net._cache
+    acts = [bf16_to_f32(a).cpu().numpy().astype(np.float64) for a in acts_t]
+    Zs = [z.cpu().numpy().astype(np.float64) for z in Zs_t]
+    assert np.array_equal(acts[0], MO.bf16(X))
     L = net.nlayers
     for l in range(L):
-        check(net.grads[l].cpu().numpy(), gW_ref[l], 'dW%d' % l)
-        check(net.grads[L + l].cpu().numpy(), gb_ref[l], 'db%d' % l)
+        Wb = MO.bf16(Wt[l])
+        Zr = acts[l] @ Wb + bt[l]
+        bound = np.abs(acts[l]) @ np.abs(Wb) + np.abs(bt[l])
+        err = np.abs(Zs[l] - Zr) / np.maximum(bound, 1e-30)
+        print('layer %d: GEMM rel err (of |A||W|) max %.2e' % (l, err.max()))
+        assert err.max() <= 2.0 ** -20, l            # FP32 accumulation over K <= 1024 terms
+        if l < L - 1:
+            hr = MO.bf16(MO.silu(Zs[l]))
+            ulp = np.abs(hr) * 2.0 ** -7 + 1e-38
+            d = np.abs(acts[l + 1] - hr)
+            print('layer %d: BF16 SiLU outputs differing %d of %d (at most %.2f ulp)'
+                  % (l, int((d > 0).sum()), d.size, float((d / ulp).max())))
+            assert (d <= ulp).all() and (d > 0).mean() <= 1e-3, l
+    # backward through the device's activations
+    w = mask.astype(np.float64)
+    dZ = 2.0 * w * (Zs[-1] - T) / w.sum()
+    for l in range(L - 1, -1, -1):
+        gW = acts[l].T @ MO.bf16(dZ)
+        gb = dZ.sum(axis=0)
+        for what, dev, ref in (('dW', net.grads[l], gW), ('db', net.grads[L + l], gb)):
+            e = np.abs(dev.cpu().numpy().astype(np.float64) - ref)
+            scale = np.abs(ref).max()
+            print('%s%d: max err %.2e of scale %.2e' % (what, l, e.max(), scale))
+            assert e.max() <= 2.0 ** -8 * scale, (what, l)
+        if l == 0:
+            break
+        dA = MO.bf16(dZ) @ MO.bf16(Wt[l]).T
+        dZ = dA * MO.dsilu(Zs[l - 1])

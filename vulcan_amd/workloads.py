@@ -154,6 +154,15 @@ def batched_cases(ncols: int = 256, net: Optional[Network] = None, nz_range=(80,
     return out
 
 
+def batched_case(index: int, ncols: int = 256, net: Optional[Network] = None, nz_range=(80, 200),
+                 seed: int = SEED_BATCH) -> ColumnCase:
+    """Column `index` of batched_cases(ncols) alone (same seed stream)."""
+    net = net or full_network()
+    c = _batched_one((np.random.SeedSequence(seed).spawn(ncols)[index], nz_range, net))
+    c.net = net
+    return c
+
+
 def _mixed_one(args):
     s, net = args
     rng = np.random.default_rng(s)
