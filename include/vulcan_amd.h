@@ -49,6 +49,7 @@ extern "C" {
 typedef struct vk_plan vk_plan;
 typedef struct vk_columns vk_columns;
 typedef struct vk_integrator vk_integrator;
+typedef struct vk_photo vk_photo;
 
 /* Host description of a network (copied into device tables by
  * vk_plan_create).  Terms: species index, -1 = M, -2 = empty slot. */
@@ -303,6 +304,68 @@ int vk_integrator_stats(const vk_integrator *it, double *factor_ms, long long *f
  * change.  No counterpart in the reference (its loop is per-step Python);
  * results are bitwise those of the eager launches. */
 int vk_integrator_set_graph(vk_integrator *it, int on);
+
+/* ---- photochemistry radiative transfer ----------------------------------
+ * Tables of ODESolver.make_bins_read_cross (op.py:495-781) and
+ * Atm.read_sflux (build_atm.py:617-657), prepared on the host: wavelength
+ * bins, absorption cross sections of the photolysed species (with per-layer
+ * temperature-dependent tables where vulcan_cfg.T_cross_sp asks for them),
+ * Rayleigh cross sections of the scattering species, and per
+ * photodissociation branch its cross section and the rate column it sets
+ * (VULCAN's pho_rate_index - 1: the reference indexes k by the id printed
+ * in the network file, op.py:241-244).  Per-layer tables are [n][L][nbin] over the stacked
+ * layers of the columns. */
+typedef struct {
+    int nbin;                  /* wavelength bins                              */
+    int nabs;                  /* absorbing (photolysed) species               */
+    int nscat;                 /* Rayleigh-scattering species (scat_sp)        */
+    int nbranch;               /* photodissociation branches                   */
+    int nT, nTJ;               /* temperature-dependent absorption / branch tables */
+    int din12;                 /* first bin of width dbin2 (-1: none)          */
+    const int *abs_sp;         /* [nabs] species index                         */
+    const int *abs_T;          /* [nabs] index into cross_T or -1              */
+    const int *scat_sp;        /* [nscat] species index                        */
+    const int *br_col;         /* [nbranch] rate column: k[l][col] = J f_diurnal */
+    const int *br_T;           /* [nbranch] index into cross_J_T or -1         */
+    const double *bins;        /* [nbin] nm                                    */
+    const double *sflux_top;   /* [nbin] stellar flux at the top               */
+    const double *cross;       /* [nabs][nbin] cm^2                            */
+    const double *cross_T;     /* [nT][L][nbin]                                */
+    const double *cross_scat;  /* [nscat][nbin]                                */
+    const double *cross_J;     /* [nbranch][nbin]                              */
+    const double *cross_J_T;   /* [nTJ][L][nbin]                               */
+    double dbin1, dbin2;       /* bin widths below / from the switch (nm)      */
+    double cos_sl;             /* cos(sl_angle)                                */
+    double edd, f_diurnal, flux_atol, hc;
+} vk_photo_desc;
+
+#define VK_PHOTO_TAU     0     /* [L + ncols][nbin] optical depth (levels)    */
+#define VK_PHOTO_SFLUX   1     /* [L + ncols][nbin] direct beam (levels)      */
+#define VK_PHOTO_DFLUX_U 2     /* [L + ncols][nbin] diffuse up flux (levels)  */
+#define VK_PHOTO_DFLUX_D 3     /* [L + ncols][nbin] diffuse down flux (levels) */
+#define VK_PHOTO_AFLUX   4     /* [L][nbin] actinic flux (layers)             */
+
+/* Uploads the tables and allocates the radiative-transfer state of the
+ * columns (column c's levels at rows off[c] + c .. off[c] + c + nz). */
+int vk_photo_create(const vk_plan *plan, const vk_columns *cols, const vk_photo_desc *desc, vk_photo **out);
+int vk_photo_destroy(vk_photo *photo);
+/* ODESolver.compute_tau + compute_flux + compute_J (op.py:2580-2787) for
+ * every column whose update is due: with force, all of them; otherwise the
+ * columns not done whose st.count is a multiple of the update frequency in
+ * force and that have not been updated at this count, the frequency going
+ * from ini_frq to final_frq for good once longdy < 10 yconv_min and
+ * longdydt < 1e-6 (op.py:818-830).  Writes the photolysis rates
+ * k[l][col] = J f_diurnal and st.aflux_change (the largest relative change
+ * of the actinic flux above flux_atol, op.py:2737), which conv() tests. */
+int vk_photo_update(vk_photo *photo, const vk_plan *plan, const vk_columns *cols, const double *d_y,
+                    const double *d_ymix, const double *d_dz, double *d_k, vk_colstate *d_st, int force,
+                    int ini_frq, int final_frq, double yconv_min, void *stream);
+/* Copies one of the VK_PHOTO_* arrays to host memory (synchronises). */
+int vk_photo_read(const vk_photo *photo, int what, double *h_out);
+/* Runs vk_photo_update at the start of every vk_integrate attempt (the top
+ * of the loop, op.py:813-830): the photolysis rows of the d_k that
+ * vk_integrate is given are rewritten in place.  NULL detaches. */
+int vk_integrator_set_photo(vk_integrator *it, vk_photo *photo, int ini_frq, int final_frq);
 
 /* ---- warm-start MLP (no counterpart in the reference: it seeds the plain
  * Ros2 integration, which stays the fallback and the judge) --------------

@@ -42,6 +42,13 @@ class Params(C.Structure):
             'refine')] + [('conv_ignore', C.c_uint * 5)]
 
 
+class PhotoDesc(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ('nbin', 'nabs', 'nscat', 'nbranch', 'nT', 'nTJ', 'din12')] + \
+        [(n, c_int_p) for n in ('abs_sp', 'abs_T', 'scat_sp', 'br_col', 'br_T')] + \
+        [(n, c_double_p) for n in ('bins', 'sflux_top', 'cross', 'cross_T', 'cross_scat', 'cross_J', 'cross_J_T')] + \
+        [(n, C.c_double) for n in ('dbin1', 'dbin2', 'cos_sl', 'edd', 'f_diurnal', 'flux_atol', 'hc')]
+
+
 COLSTATE_DTYPE = np.dtype([
     ('t', 'f8'), ('dt', 'f8'), ('delta', 'f8'), ('longdy', 'f8'), ('longdydt', 'f8'), ('small_y', 'f8'),
     ('nega_y', 'f8'), ('aflux_change', 'f8'), ('count', 'i4'), ('attempts', 'i4'), ('nega_count', 'i4'),
@@ -79,6 +86,11 @@ _SIGS = {
                      C.c_int, VP],
     'vk_conv': [VP, VP, C.POINTER(Atm), C.POINTER(Params), VP, VP, VP, C.c_int, VP, C.c_int, VP, VP],
     'vk_integrator_history': [VP, C.c_int, C.c_int, C.c_int, VP, VP],
+    'vk_photo_create': [VP, VP, C.POINTER(PhotoDesc), C.POINTER(VP)],
+    'vk_photo_destroy': [VP],
+    'vk_photo_update': [VP, VP, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_double, VP],
+    'vk_photo_read': [VP, C.c_int, VP],
+    'vk_integrator_set_photo': [VP, VP, C.c_int, C.c_int],
     'vk_update_mu_dz': [VP, VP, C.POINTER(Atm), C.POINTER(Geom), VP, VP, C.c_int, VP],
     'vk_atom_sums': [VP, VP, VP, VP, VP],
     'vk_integrator_set_timing': [VP, C.c_int],

@@ -1,10 +1,10 @@
 """Numerical knobs, with VULCAN's names and defaults (vulcan_cfg.py:124-159).
 
 ``Config`` is a plain attribute bag so that host code reads the same names the
-reference reads from ``vulcan_cfg``.  Photochemistry radiative transfer,
-condensation, ions, fixed-species and flux boundary conditions are outside the
-accelerated path; their switches exist only so that a configuration asking for
-them is rejected loudly instead of being silently ignored.
+reference reads from ``vulcan_cfg``.  Condensation, ions, fixed-species and flux
+boundary conditions are outside the accelerated path; their switches exist
+only so that a configuration asking for them is rejected loudly instead of
+being silently ignored.
 """
 from __future__ import annotations
 
@@ -60,6 +60,18 @@ class Config:
     flux_cri: float = 0.1
     conver_ignore: List[str] = dataclasses.field(default_factory=list)
     rtol: float = 0.25                # vulcan_cfg.py:156
+    # photochemistry (vulcan_cfg.py:43-60, 151); the radiative transfer runs
+    # on the device when the caller provides its tables (vulcan_amd.photo)
+    use_photo: bool = False
+    ini_update_photo_frq: int = 100
+    final_update_photo_frq: int = 5
+    f_diurnal: float = 1.
+    sl_angle: float = 48 / 180. * 3.14159
+    edd: float = 0.5
+    flux_atol: float = 1.
+    scat_sp: List[str] = dataclasses.field(default_factory=lambda: ['H2', 'He'])
+    T_cross_sp: List[str] = dataclasses.field(default_factory=list)
+    remove_list: List[int] = dataclasses.field(default_factory=list)
     # iterative-refinement steps per block solve (double-double residual);
     # no counterpart in the reference, whose banded LU needs none
     refine: int = 1

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 SOURCES = ['vk_solve.hip', 'vk_factor_reg.hip', 'vk_factor_blk1.hip', 'vk_factor_blk2.hip', 'vk_factor_wave.hip', 'vk_bcr.hip',
-           'vk_chem.hip', 'vk_ros2.hip', 'vk_mlp.hip']
+           'vk_chem.hip', 'vk_ros2.hip', 'vk_mlp.hip', 'vk_photo.hip']
 TARGET = os.path.join(PKG, 'libvulcan_amd.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 

@@ -72,6 +72,11 @@ struct vk_columns {
 // per-layer partial results of the finalize kernel
 enum { VKP_DELTA = 0, VKP_NEG = 1, VKP_SMALL = 2, VKP_NEGY = 3, VKP_ATOM = 4 };
 
+// photochemistry radiative transfer of a set of columns (vk_photo.hip)
+int vk_launch_photo(vk_photo *p, const vk_plan *plan, const vk_columns *cols, const double *y, const double *ymix,
+                    const double *dz, double *k, vk_colstate *st, int force, int ini_frq, int final_frq,
+                    double yconv_min, hipStream_t s);
+
 struct vk_integrator {
     const vk_plan *plan;
     const vk_columns *cols;
@@ -96,6 +101,8 @@ struct vk_integrator {
     hipGraphExec_t gexec;
     hipStream_t gstream;
     std::vector<char> gkey;    // arguments baked into gexec
+    vk_photo *photo;           // radiative transfer refreshed in the loop (may be null)
+    int photo_ini_frq, photo_final_frq;
     std::vector<void *> allocs;
 };
 

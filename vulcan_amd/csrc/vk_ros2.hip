@@ -671,6 +671,8 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
     it->gstream = nullptr;
     it->factor_ms = 0.0;
     it->factor_launches = 0;
+    it->photo = nullptr;
+    it->photo_ini_frq = it->photo_final_frq = 0;
     *out = it;
     return 0;
 }
@@ -800,7 +802,15 @@ static int loop_body(vk_integrator *it, const vk_atm *atm, const vk_geom *g, con
                      double *d_atom_loss, double *d_atom_loss_prev, vk_colstate *d_st, bool timed, hipStream_t s) {
     const vk_plan *p = it->plan;
     const vk_columns *cl = it->cols;
-    int r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, timed, s);
+    int r = 0;
+    if (it->photo) {
+        // compute_tau / flux / J at the top of the loop (op.py:818-830); the
+        // photolysis rows of k are the ones the attempt then reads
+        r = vk_launch_photo(it->photo, p, cl, d_y, d_ymix, g->dz, const_cast<double *>(d_k), d_st, 0,
+                            it->photo_ini_frq, it->photo_final_frq, prm->yconv_min, s);
+        if (r) return r;
+    }
+    r = ros2_attempt(it, atm, prm, d_k, d_y, d_ymix, d_st, 1, timed, s);
     if (r) return r;
     const size_t dsm = (size_t)cl->max_nz * NP * sizeof(double);
     const int stage = dsm <= DECIDE_STAGE_MAX;
@@ -824,6 +834,8 @@ struct GraphKey {
     vk_params prm;
     const void *ptr[9];
     int factor_variant;
+    const void *photo;
+    int photo_frq[2];
 };
 
 extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom *geom, const vk_params *prm,
@@ -833,7 +845,7 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     if (!it || !atm || !prm || !d_k || !d_y || !d_ymix || !d_y_prev || !d_atom_ini || !d_atom_loss ||
         !d_atom_loss_prev || !d_st || n_attempts < 0)
         return VK_EINVAL;
-    if (prm->use_update_mu && !geom) return VK_EINVAL;
+    if ((prm->use_update_mu || it->photo) && !geom) return VK_EINVAL;
     if (prm->use_conv && it->history > 0 && it->history < prm->conv_step + 1) return VK_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     int r = ensure_thist(it, prm->count_max);
@@ -860,6 +872,9 @@ extern "C" int vk_integrate(vk_integrator *it, const vk_atm *atm, const vk_geom 
     const void *ptrs[9] = {d_k, d_y, d_ymix, d_y_prev, d_atom_ini, d_atom_loss, d_atom_loss_prev, d_st, it->thist};
     memcpy(key.ptr, ptrs, sizeof(ptrs));
     key.factor_variant = it->plan->factor_variant;
+    key.photo = it->photo;
+    key.photo_frq[0] = it->photo_ini_frq;
+    key.photo_frq[1] = it->photo_final_frq;
     if (!it->gstream) VK_CHECK(hipStreamCreateWithFlags(&it->gstream, hipStreamNonBlocking));
     if (!it->gexec || it->gkey.size() != sizeof(key) || memcmp(it->gkey.data(), &key, sizeof(key)) != 0) {
         if (it->gexec) { (void)hipGraphExecDestroy(it->gexec); it->gexec = nullptr; }
@@ -919,6 +934,14 @@ extern "C" int vk_integrator_history(vk_integrator *it, int write, int count, in
             VK_CHECK(write ? hipMemcpy(dv, hv, sizeof(double) * Lni, dir) : hipMemcpy(hv, dv, sizeof(double) * Lni, dir));
         }
     }
+    return 0;
+}
+
+extern "C" int vk_integrator_set_photo(vk_integrator *it, vk_photo *photo, int ini_frq, int final_frq) {
+    if (!it || (photo && (ini_frq < 1 || final_frq < 1))) return VK_EINVAL;
+    it->photo = photo;
+    it->photo_ini_frq = ini_frq;
+    it->photo_final_frq = final_frq;
     return 0;
 }
 

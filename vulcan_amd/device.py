@@ -277,6 +277,21 @@ class ColumnBatch:
                    'vk_tune_factor')
         return list(ms)
 
+    def attach_photo(self, photo, ini_frq: int, final_frq: int):
+        """Refresh the photolysis rates (and the actinic-flux change conv()
+        tests) at the top of the loop every ini_frq steps, then every
+        final_frq steps once the column is near steady state (op.py:818-830);
+        photo is a vulcan_amd.photo.DevicePhoto of this batch's layout, or
+        None."""
+        self.photo = photo
+        _lib.check(self._lib.vk_integrator_set_photo(self.it, photo.handle if photo is not None else None,
+                                                     int(ini_frq), int(final_frq)), 'vk_integrator_set_photo')
+
+    def update_photo(self, force: bool = True):
+        """compute_tau + compute_flux + compute_J now (vulcan.py:172-174)."""
+        self.photo.update(self.y, self.ymix, self.dz, self.k, state=self.state, force=force,
+                          yconv_min=self.cfg.yconv_min)
+
     def set_graph(self, on: bool):
         """Replay a captured HIP graph per attempt in ``integrate``."""
         _lib.check(self._lib.vk_integrator_set_graph(self.it, int(on)), 'vk_integrator_set_graph')

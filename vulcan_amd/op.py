@@ -16,9 +16,10 @@ argument meaning on the same ``var``/``atm``/``para`` attribute names
   (backup, one_step, update_mu_dz, renormalisation, save_step, step_size,
   stop/conv) resident on the device, polled every ``poll`` attempts.
 
-Photochemical radiative transfer (``compute_tau/flux/J``), condensation,
-ions and flux boundary conditions are not part of the accelerated path and
-raise ``NotImplementedError``; photolysis rates enter as prescribed ``k``.
+``compute_tau`` / ``compute_flux`` / ``compute_J`` run the photochemical
+radiative transfer on the device from the tables VULCAN's set-up left in
+``var``.  Condensation, ions and flux boundary conditions are not part of
+the accelerated path and raise ``NotImplementedError``.
 Numerical failure is never raised (negative y, delta and element loss are
 reported through ``var``/``para``, as in the reference).
 """
@@ -68,11 +69,63 @@ class Ros2:
         """op.py:3078-3088."""
         para.solver_str = 'solver'
 
-    def compute_tau(self, var, atm):
-        raise NotImplementedError('photochemical radiative transfer is outside the accelerated path; '
-                                  'prescribe photolysis rates in var.k')
+    # -- photochemistry (op.py:2580-2787) on the device ----------------------------
+    def _photo(self, var, atm):
+        """The device radiative transfer of this solver's column, built from
+        the tables VULCAN's set-up left in var (make_bins_read_cross,
+        read_sflux) the first time."""
+        from .photo import DevicePhoto, tables_from_var
+        nz = np.shape(var.y)[0]
+        b = self._device_batch(atm, nz)
+        if getattr(self, '_photo_dev', None) is None or self._photo_nz != nz:
+            self._photo_dev = DevicePhoto(b.plan, b.layout, tables_from_var(var, self.net.species, self.net.nr,
+                                                                            self.cfg))
+            self._photo_nz = nz
+        return b, self._photo_dev
 
-    compute_flux = compute_J = compute_Jion = compute_tau
+    def compute_tau(self, var, atm):
+        """compute_tau + compute_flux + compute_J in one device update
+        (op.py:2580-2787); var.tau and the fluxes are filled here,
+        var.k[...] and var.J_sp by compute_J."""
+        import torch
+        from . import photo as P
+        b, ph = self._photo(var, atm)
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64))
+        b.y.copy_(T(var.y))
+        b.ymix.copy_(T(var.ymix))
+        b.dz.copy_(T(atm.dz))
+        b.k.copy_(T(k_dict_to_array(var.k, np.shape(var.y)[0], self.net.nr)))
+        st = b.host_state()
+        st['aflux_change'] = 0.0
+        b.set_host_state(st)
+        ph.update(b.y, b.ymix, b.dz, b.k, state=b.state, force=True)
+        var.tau = ph.read(P.VK_PHOTO_TAU)
+        var.sflux = ph.read(P.VK_PHOTO_SFLUX)
+        var.dflux_u = ph.read(P.VK_PHOTO_DFLUX_U)
+        var.dflux_d = ph.read(P.VK_PHOTO_DFLUX_D)
+        var.prev_aflux = np.copy(getattr(var, 'aflux', np.zeros(1)))
+        var.aflux = ph.read(P.VK_PHOTO_AFLUX)
+        var.aflux_change = float(b.host_state()['aflux_change'][0])
+        self._photo_k = b.k.cpu().numpy()
+
+    def compute_flux(self, var, atm):
+        """Done by compute_tau's device update (op.py:2603-2738)."""
+
+    def compute_J(self, var, atm):
+        """Photolysis rates of the last device update into var.k (op.py:2751-2787):
+        k = J f_diurnal at the reference's pho_rate_index, var.J_sp[(sp, br)] = J."""
+        t = self._photo_dev.tables
+        cols = set(t.br_col)
+        for sp_br, idx in var.pho_rate_index.items():
+            if idx - 1 in cols:
+                var.k[idx] = self._photo_k[:, idx - 1].copy()
+        var.J_sp = {}
+        for sp_br, idx in var.pho_rate_index.items():
+            if idx - 1 in cols:
+                var.J_sp[sp_br] = self._photo_k[:, idx - 1] / self.cfg.f_diurnal
+
+    def compute_Jion(self, var, atm):
+        raise NotImplementedError('photoionisation (use_ion) is outside the accelerated path')
 
     # -- the device attempt ------------------------------------------------------
     def _device_batch(self, atm, nz):
@@ -246,7 +299,20 @@ class Integration:
             b.write_history(np.array(t_hist)[None], np.stack(y_hist[len(y_hist) - (count0 - q0):]), q0)
         st = b.host_state()
         st['dt'], st['t'], st['count'] = var.dt, var.t, count0
+        st['longdy'], st['longdydt'] = getattr(var, 'longdy', 1.0), getattr(var, 'longdydt', 1.0)
+        st['aflux_change'] = getattr(var, 'aflux_change', 0.0)
         b.set_host_state(st)
+        if cfg.use_photo:
+            # the radiative transfer of the solver's own set-up call
+            # (vulcan.py:172-174) carries on inside the loop; without one,
+            # the tables come from var and the set-up call is made here
+            ph = getattr(solver, '_photo_dev', None)
+            if ph is None or solver._photo_nz != nz:
+                from .photo import DevicePhoto, tables_from_var
+                ph = DevicePhoto(plan, b.layout, tables_from_var(var, solver.net.species, solver.net.nr, cfg))
+                b.photo = ph
+                b.update_photo(force=True)
+            b.attach_photo(ph, cfg.ini_update_photo_frq, cfg.final_update_photo_frq)
         poll = max(1, min(self.poll, H))
         seen = count0
         y_time = y_hist if self.keep_y_time else []
@@ -264,6 +330,13 @@ class Integration:
         var.t_time = [float(t) for t in t_all[0]]
         var.y_time = y_time if self.keep_y_time else [ys[q] for q in range(ys.shape[0])]
         var.y, var.ymix = b.y.cpu().numpy(), b.ymix.cpu().numpy()
+        if cfg.use_photo:
+            kk = b.k.cpu().numpy()
+            for idx in var.pho_rate_index.values():
+                if 1 <= idx <= solver.net.nr:
+                    var.k[idx] = kk[:, idx - 1].copy()
+            var.aflux_change = float(h['aflux_change'])
+            b.attach_photo(None, 1, 1)
         var.t, var.dt = float(h['t']), float(h['dt'])
         var.longdy, var.longdydt = float(h['longdy']), float(h['longdydt'])
         para.count, para.end_case, para.delta = int(h['count']), int(h['end_case']), float(h['delta'])
