@@ -37,7 +37,10 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 NZ = 8
 
 
-def main():
+def setup():
+    """Imports the reference and builds the seeded column of the generated
+    network (rates consistent with its K_eq); returns the objects main()
+    and the other generators work on."""
     sys.dont_write_bytecode = True
     os.environ.setdefault('MPLBACKEND', 'Agg')
     os.chdir(REF)
@@ -130,6 +133,20 @@ def main():
 
     solver = op.Ros2()
     solver.naming_solver(para)
+    return dict(cfg=cfg, chem_funs=chem_funs, store=store, build_atm=build_atm, op=op, rng=rng, species=species,
+                ni=ni, nr=nr, R=R, sp_idx=sp_idx, reac=reac, prod=prod, nasa9=nasa9, elements=elements, compo=compo,
+                mass=mass, Tg=Tg, Kg=Kg, var=var, atm=atm, para=para, make_atm=make_atm, geom0=geom0, ini=ini,
+                karr=karr, solver=solver)
+
+
+def main():
+    ctx = setup()
+    cfg, chem_funs, store, build_atm, op = ctx['cfg'], ctx['chem_funs'], ctx['store'], ctx['build_atm'], ctx['op']
+    rng, species, ni, nr, R, sp_idx = ctx['rng'], ctx['species'], ctx['ni'], ctx['nr'], ctx['R'], ctx['sp_idx']
+    reac, prod, nasa9, elements, compo, mass = (ctx[k] for k in ('reac', 'prod', 'nasa9', 'elements', 'compo',
+                                                                 'mass'))
+    Tg, Kg, var, atm, para, make_atm = ctx['Tg'], ctx['Kg'], ctx['var'], ctx['atm'], ctx['para'], ctx['make_atm']
+    geom0, ini, karr, solver = ctx['geom0'], ctx['ini'], ctx['karr'], ctx['solver']
     gold = {}
     y0 = var.y.copy()
     gold['chemdf'] = chem_funs.chemdf(y0, atm.M, var.k)

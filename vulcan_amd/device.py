@@ -216,6 +216,7 @@ class ColumnBatch:
         self.atom_loss_prev = torch.zeros_like(self.atom_ini)
         st = np.zeros(nc, _lib.COLSTATE_DTYPE)
         st['dt'] = cfg.dttry
+        st['longdy'] = st['longdydt'] = 1.0      # store.Variables' initial values (store.py:36-38)
         self.state = torch.as_tensor(st.view(np.uint8).copy(), device=dev)
         self.history = history
         lib = _lib.load()
