@@ -36,7 +36,8 @@ extern "C" {
 #define VK_ETOOBIG  (-1003)   /* ni beyond the compiled maximum  */
 
 #define VK_NI_LIMIT 160           /* largest ni the kernels are compiled for */
-#define VK_CONV_IGNORE_WORDS 5    /* ceil(VK_NI_LIMIT / 32)                   */
+#define VK_SPECIES_WORDS 5        /* ceil(VK_NI_LIMIT / 32): one bit per species */
+#define VK_CONV_IGNORE_WORDS VK_SPECIES_WORDS
 
 /* rate-expression kinds (par[p][0..7] = A n E Ainf ninf Einf Fc J0) */
 #define VK_KIND_ARRH   0  /* op.py:166               */
@@ -94,6 +95,25 @@ typedef struct {
     double *g;              /* layers (updated)                        */
     double *Hp;             /* layers (updated)                        */
     const double *tau;      /* photolysis optical depth, may be NULL   */
+    /* Transport terms and boundary conditions beyond the default
+     * configuration; a NULL array or a zero switch turns each off. */
+    const double *vz;       /* interfaces: vertical velocity, upwind advection (use_vz; op.py:1545-1570) */
+    const double *vm;       /* [L][ni] layers: molecular-diffusion velocity; selects the upwind
+                               molecular terms of diffdf_vm / lhs_jac_tot_vm (use_vm_mol; op.py:1599-1694,
+                               2044-2120)                                                               */
+    double *top_flux;       /* [ncols][ni] top flux, cm^-2 s^-1 (store.py:138); rewritten for the
+                               escaping species every update_frq steps (op.py:986-999)                  */
+    const double *bot_flux; /* [ncols][ni] bottom flux (store.py:139)                                   */
+    const double *bot_vdep; /* [ncols][ni] deposition velocity, cm s^-1 (store.py:140)                  */
+    const double *fix_bot;  /* [ncols][ni] fixed bottom mixing ratio, < 0: free (use_fix_sp_bot;
+                               op.py:2945-2946); NULL: none                                             */
+    int use_topflux;        /* diffdf adds top_flux / dzi[-1] (op.py:1589-1592)                         */
+    int use_botflux;        /* diffdf adds (bot_flux - y bot_vdep) / dzi[0], the lhs the deposition
+                               (op.py:1593-1596, 2029); delta skips layer 0 (op.py:2953)               */
+    /* diff_esc (vulcan_cfg.py:101): bit (i % 32) of word i / 32 set = species i
+     * escapes at the diffusion limit, capped at max_flux (vulcan_cfg.py:102) */
+    unsigned int esc[VK_SPECIES_WORDS];
+    double max_flux;
 } vk_atm;
 
 /* Column geometry inputs for update_mu_dz (per column / per layer). */
@@ -287,6 +307,11 @@ int vk_integrator_history(vk_integrator *it, int write, int count, int q0,
 int vk_update_mu_dz(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
                     const vk_geom *geom, const double *d_ymix, const vk_colstate *d_st,
                     int force, void *stream);
+/* update_phi_esc (op.py:986-999): top_flux[c][i] = max(-Dzz[-1][i] y[-1][i]
+ * (1/Hp[-1] - ms[i] g[-1] / (Navo kb T[-1])), -max_flux) for the species of
+ * atm->esc, every column (no-op when none is marked) */
+int vk_update_phi_esc(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm,
+                      const double *d_y, void *stream);
 /* element sums: atom_sum[c][a] = sum_{l,i} atom_mat[i][a] y[l][i] (op.py:2472-2487, build_atm.py:283-291) */
 int vk_atom_sums(const vk_plan *plan, const vk_columns *cols, const double *d_y,
                  double *d_atom_sum, void *stream);

@@ -11,8 +11,12 @@ kernels can be compared nearly bit-for-bit:
 * rate constants                      op.py:145-207, 287-309; chem_funs.py:2769-2796
 * ``chemdf``                          make_chem_funs.py:199-316 (generated chem_funs.py:931-1000)
 * analytic Jacobian (``symjac``)      make_chem_funs.py:653-717
-* ``diffdf`` / ``diffdf_no_mol``      op.py:1496-1597 / 1438-1494
-* ``lhs_jac_tot`` / ``lhs_jac_no_mol`` op.py:1973-2042 / 2122-2166
+* ``diffdf`` / ``diffdf_vm`` / ``diffdf_no_mol``   op.py:1496-1597 / 1599-1694 / 1438-1494,
+  with upwind vertical advection and the top/bottom flux terms
+* ``lhs_jac_tot`` / ``lhs_jac_tot_vm`` / ``lhs_jac_no_mol``   op.py:1973-2042 / 2044-2120 / 2122-2166,
+  with the deposition velocity and the diffusion-limited escape term
+* fixed bottom mixing ratios, delta without layer 0   op.py:1435-1436, 2945-2953
+* ``update_phi_esc``                  op.py:986-999
 * ``store_bandM`` + two ``solve_banded`` calls   op.py:2833-2858, 2914, 2929
 * ``Ros2.solver``                     op.py:2860-3007
 * ``clip``/``loss``/``step_ok``/``step_reject``/``reset_y``  op.py:2447-2534
@@ -221,50 +225,102 @@ def _drift(col, j_int, gl):
         + col.alpha / Ti[j_int] * (Tco[j_int + 1] - Tco[j_int]) / dzi[j_int]
 
 
-def diffdf(col, y, use_moldiff=True) -> np.ndarray:
-    """op.py:1496-1597 (use_moldiff) or op.py:1438-1494; vz = 0, no flux BCs."""
+def _vp(v):
+    """(v > 0) * v: the upwind split of an advection velocity (op.py:1545-1548)."""
+    return (v > 0) * v
+
+
+def _vn(v):
+    """(v < 0) * v."""
+    return (v < 0) * v
+
+
+def _bc(cfg):
+    """(use_vm_mol, use_topflux, use_botflux) of a configuration (None: all off)."""
+    if cfg is None:
+        return False, False, False
+    return bool(cfg.use_vm_mol), bool(cfg.use_topflux), bool(cfg.use_botflux)
+
+
+def diffdf(col, y, use_moldiff=True, cfg=None) -> np.ndarray:
+    """op.py:1496-1597 (use_moldiff), op.py:1599-1694 (use_moldiff and
+    cfg.use_vm_mol) or op.py:1438-1494.  Vertical advection uses col.vz
+    (zeros unless use_vz); cfg switches the constant top flux and the bottom
+    flux with deposition velocity (col.top_flux, col.bot_flux, col.bot_vdep)."""
+    use_vm, use_top, use_bot = _bc(cfg)
     nz, ni = y.shape
     ysum = np.sum(y, axis=1)
-    dzi, Kzz, Dzz, g = col.dzi, col.Kzz, col.Dzz, col.g
+    dzi, Kzz, Dzz, g, vz = col.dzi, col.Kzz, col.Dzz, col.g, col.vz
     A, B, C = np.zeros(nz), np.zeros(nz), np.zeros(nz)
     A[0] = -1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0]
     B[0] = 1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1]
     A[nz - 1] = -1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1]
     C[nz - 1] = 1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2]
+    A[0] += -_vp(vz[0]) / dzi[0]
+    B[0] += -_vn(vz[0]) / dzi[0]
+    A[-1] += _vn(vz[-1]) / dzi[-1]
+    C[-1] += _vp(vz[-1]) / dzi[-1]
     j = np.arange(1, nz - 1)
     dz_ave = 0.5 * (dzi[j - 1] + dzi[j])
     A[j] = -1. / dz_ave * (Kzz[j] / dzi[j] * (ysum[j + 1] + ysum[j]) / 2. + Kzz[j - 1] / dzi[j - 1] * (ysum[j] + ysum[j - 1]) / 2.) / ysum[j]
     B[j] = 1. / dz_ave * Kzz[j] / dzi[j] * (ysum[j + 1] + ysum[j]) / 2. / ysum[j + 1]
     C[j] = 1. / dz_ave * Kzz[j - 1] / dzi[j - 1] * (ysum[j] + ysum[j - 1]) / 2. / ysum[j - 1]
+    A[j] += -(_vp(vz[j]) - _vn(vz[j - 1])) / dz_ave
+    B[j] += -_vn(vz[j]) / dz_ave
+    C[j] += _vp(vz[j - 1]) / dz_ave
+    diff = np.empty_like(y)
     if not use_moldiff:
-        diff = np.empty_like(y)
         diff[0] = A[0] * y[0] + B[0] * y[1]
         diff[1:nz - 1] = A[j, None] * y[j] + B[j, None] * y[j + 1] + C[j, None] * y[j - 1]
         diff[nz - 1] = A[nz - 1] * y[nz - 1] + C[nz - 1] * y[nz - 2]
-        return diff
+        return _flux_bc(col, y, diff, use_top, use_bot)
     Ai = np.zeros((nz, ni)); Bi = np.zeros((nz, ni)); Ci = np.zeros((nz, ni))
-    d0 = _drift(col, 0, g[0])
-    Ai[0] = -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
-    Bi[0] = 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
-    dt_ = _drift(col, nz - 2, g[nz - 1])
-    Ai[nz - 1] = -1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1] \
-        - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
-    Ci[nz - 1] = 1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2] \
-        - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
+    if use_vm:
+        vm = col.vm
+        Ai[0] = -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0] - _vp(vm[0]) / dzi[0]
+        Bi[0] = 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1] - _vn(vm[0]) / dzi[0]
+        Ai[nz - 1] = -1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1] \
+            + _vn(vm[-1]) / dzi[-1]
+        Ci[nz - 1] = 1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2] \
+            + _vp(vm[-1]) / dzi[-1]
+    else:
+        d0 = _drift(col, 0, g[0])
+        Ai[0] = -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[0] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+        Bi[0] = 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / 2. / ysum[1] + 1. / (dzi[0]) * Dzz[0] / 2. * d0
+        dt_ = _drift(col, nz - 2, g[nz - 1])
+        Ai[nz - 1] = -1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 1] \
+            - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
+        Ci[nz - 1] = 1. / (dzi[-1]) * (Dzz[nz - 2] / dzi[-1]) * (ysum[nz - 1] + ysum[nz - 2]) / 2. / ysum[nz - 2] \
+            - 1. / (dzi[-1]) * Dzz[-1] / 2. * dt_
     for jj in range(1, nz - 1):
         da = 0.5 * (dzi[jj - 1] + dzi[jj])
         Ai[jj] = -1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj] + ysum[jj - 1]) / 2.) / ysum[jj]
         Bi[jj] = 1. / da * Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. / ysum[jj + 1]
         Ci[jj] = 1. / da * Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj] + ysum[jj - 1]) / 2. / ysum[jj - 1]
-        Ai[jj] += 1. / (2. * da) * (Dzz[jj] * _drift(col, jj, g[jj]) - Dzz[jj - 1] * _drift(col, jj - 1, g[jj]))
-        Bi[jj] += 1. / (2. * da) * Dzz[jj] * _drift(col, jj, g[jj + 1])
-        Ci[jj] += -1. / (2. * da) * Dzz[jj - 1] * _drift(col, jj - 1, g[jj - 1])
-    diff = np.empty_like(y)
+        if use_vm:
+            # upwind molecular velocity on the layer rows of vm (op.py:1666-1670)
+            Ai[jj] += -(_vp(vm[jj]) - _vn(vm[jj - 1])) / da
+            Bi[jj] += -_vn(vm[jj]) / da
+            Ci[jj] += +_vp(vm[jj - 1]) / da
+        else:
+            Ai[jj] += 1. / (2. * da) * (Dzz[jj] * _drift(col, jj, g[jj]) - Dzz[jj - 1] * _drift(col, jj - 1, g[jj]))
+            Bi[jj] += 1. / (2. * da) * Dzz[jj] * _drift(col, jj, g[jj + 1])
+            Ci[jj] += -1. / (2. * da) * Dzz[jj - 1] * _drift(col, jj - 1, g[jj - 1])
     diff[0] = (A[0] + Ai[0]) * y[0] + (B[0] + Bi[0]) * y[1]
     mid = A[j, None] * y[j] + B[j, None] * y[j + 1] + C[j, None] * y[j - 1]
     mid += Ai[j] * y[j] + Bi[j] * y[j + 1] + Ci[j] * y[j - 1]
     diff[1:nz - 1] = mid
     diff[nz - 1] = (A[nz - 1] + Ai[nz - 1]) * y[nz - 1] + (C[nz - 1] + Ci[nz - 1]) * y[nz - 2]
+    return _flux_bc(col, y, diff, use_top, use_bot)
+
+
+def _flux_bc(col, y, diff, use_top, use_bot):
+    """Constant top flux and bottom flux minus deposition (op.py:1589-1596)."""
+    dzi = col.dzi
+    if use_top:
+        diff[-1] += col.top_flux / dzi[-1]
+    if use_bot:
+        diff[0] += (col.bot_flux - y[0] * col.bot_vdep) / dzi[0]
     return diff
 
 
@@ -272,16 +328,23 @@ def diffdf(col, y, use_moldiff=True) -> np.ndarray:
 # A5: Ros2 left-hand side as blocks
 # ---------------------------------------------------------------------------
 
-def lhs_blocks(col, y, dt, J, use_moldiff=True):
-    """lhs = c0 I - J_chem - J_diff (op.py:1973-2042 / 2122-2166).
+def lhs_blocks(col, y, dt, J, use_moldiff=True, cfg=None, esc=None):
+    """lhs = c0 I - J_chem - J_diff (op.py:1973-2042, 2044-2120 with
+    cfg.use_vm_mol, 2122-2166 without molecular diffusion).
 
     Returns D (nz, ni, ni) diagonal blocks, Lsub (nz, ni) with Lsub[j] the
     diagonal of block (j, j-1) and Usup (nz, ni) with Usup[j] the diagonal
-    of block (j, j+1); Lsub[0] = Usup[nz-1] = 0.
+    of block (j, j+1); Lsub[0] = Usup[nz-1] = 0.  cfg.use_botflux adds the
+    deposition velocity col.bot_vdep to the bottom diagonal; with the _vm
+    operators, the species indices ``esc`` (diff_esc) take
+    top_flux / y on the top diagonal.
     """
+    use_vm, _, use_bot = _bc(cfg)
+    use_vm = use_vm and use_moldiff
     nz, ni = y.shape
     ysum = np.sum(y, axis=1)
-    dzi, Kzz, Dzz, g = col.dzi, col.Kzz, col.Dzz, col.g
+    dzi, Kzz, Dzz, g, vz = col.dzi, col.Kzz, col.Dzz, col.g, col.vz
+    vm = col.vm if use_vm else None
     c0 = 1. / (R_ROS * dt)
     D = -J.copy()
     idx = np.arange(ni)
@@ -289,33 +352,60 @@ def lhs_blocks(col, y, dt, J, use_moldiff=True):
     L = np.zeros((nz, ni)); U = np.zeros((nz, ni))
     for jj in range(1, nz - 1):
         da = 0.5 * (dzi[jj - 1] + dzi[jj])
-        D[jj, idx, idx] -= -1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj]
-        U[jj] -= 1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1]))
-        L[jj] -= 1. / da * (Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1]))
-        if use_moldiff:
+        D[jj, idx, idx] -= -1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj] \
+            - (_vp(vz[jj]) - _vn(vz[jj - 1])) / da
+        U[jj] -= 1. / da * (Kzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1])) - _vn(vz[jj]) / da
+        L[jj] -= 1. / da * (Kzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1])) + _vp(vz[jj - 1]) / da
+        if use_vm:
+            D[jj, idx, idx] -= -1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj] \
+                - (_vp(vm[jj]) - _vn(vm[jj - 1])) / da
+            U[jj] -= 1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1])) - _vn(vm[jj]) / da
+            L[jj] -= 1. / da * (Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1])) + _vp(vm[jj - 1]) / da
+        elif use_moldiff:
             D[jj, idx, idx] -= -1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / 2. + Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / 2.) / ysum[jj] \
                 + 1. / (2. * da) * (Dzz[jj] * _drift(col, jj, g[jj]) - Dzz[jj - 1] * _drift(col, jj - 1, g[jj]))
             U[jj] -= 1. / da * (Dzz[jj] / dzi[jj] * (ysum[jj + 1] + ysum[jj]) / (2. * ysum[jj + 1])) \
                 + 1. / (2. * da) * Dzz[jj] * _drift(col, jj, g[jj + 1])
             L[jj] -= 1. / da * (Dzz[jj - 1] / dzi[jj - 1] * (ysum[jj - 1] + ysum[jj]) / (2. * ysum[jj - 1])) \
                 - 1. / (2. * da) * Dzz[jj - 1] * _drift(col, jj - 1, g[jj - 1])
-    D[0, idx, idx] -= -1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0])
-    if use_moldiff:
+    D[0, idx, idx] -= -1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0]) - _vp(vz[0]) / dzi[0]
+    if use_vm:
+        D[0, idx, idx] -= -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0]) - _vp(vm[0]) / dzi[0]
+    elif use_moldiff:
         d0 = _drift(col, 0, g[0])
         D[0, idx, idx] -= -1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[0]) \
             + 1. / (dzi[0]) * Dzz[0] / 2. * d0
-    U[0] -= 1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1])
-    if use_moldiff:
+    if use_bot:
+        D[0, idx, idx] -= -1. * col.bot_vdep / dzi[0]
+    if use_vm and esc is not None and len(esc):
+        # diffusion-limited escape in the Jacobian (op.py:2101-2107)
+        diff_lim = np.zeros(ni)
+        for i in esc:
+            if y[-1, i] > 0:
+                diff_lim[i] += col.top_flux[i] / y[-1, i]
+        D[nz - 1, idx, idx] -= diff_lim
+    U[0] -= 1. / (dzi[0]) * (Kzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1]) - _vn(vz[0]) / dzi[0]
+    if use_vm:
+        U[0] -= 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1]) - _vn(vm[0]) / dzi[0]
+    elif use_moldiff:
         U[0] -= 1. / (dzi[0]) * (Dzz[0] / dzi[0]) * (ysum[1] + ysum[0]) / (2. * ysum[1]) \
             + 1. / (dzi[0]) * Dzz[0] / 2. * d0
     t = nz - 1
-    D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t])
-    if use_moldiff:
+    D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t]) \
+        + _vn(vz[-1]) / dzi[-1]
+    if use_vm:
+        D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t]) \
+            + _vn(vm[-1]) / dzi[-1]
+    elif use_moldiff:
         dtop = _drift(col, nz - 2, g[t])
         D[t, idx, idx] -= -1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t]) \
             - 1. / (dzi[-1]) * Dzz[-1] / 2. * dtop
-    L[t] -= 1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t - 1])
-    if use_moldiff:
+    L[t] -= 1. / (dzi[nz - 2]) * (Kzz[nz - 2] / dzi[nz - 2]) * (ysum[t - 1] + ysum[t]) / (2. * ysum[t - 1]) \
+        + _vp(vz[-1]) / dzi[-1]
+    if use_vm:
+        L[t] -= 1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t - 1]) \
+            + _vp(vm[-1]) / dzi[-1]
+    elif use_moldiff:
         L[t] -= 1. / (dzi[nz - 2]) * (Dzz[nz - 2] / dzi[nz - 2]) * (ysum[t] + ysum[nz - 2]) / (2. * ysum[t - 1]) \
             - 1. / (dzi[-1]) * Dzz[-1] / 2. * dtop
     return D, L, U
@@ -482,9 +572,10 @@ def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False, 
     y, ymix, h, k = st.y, st.ymix, st.dt, st.k
     M = col.M
     r = R_ROS
-    df = chemdf(net, y, M, k, tables).flatten() + diffdf(col, y, cfg.use_moldiff).flatten()
+    esc = esc_indices(net.species, cfg)
+    df = chemdf(net, y, M, k, tables).flatten() + diffdf(col, y, cfg.use_moldiff, cfg).flatten()
     J = jac_blocks(net, y, M, k, tables)
-    D, L, U = lhs_blocks(col, y, h, J, cfg.use_moldiff)
+    D, L, U = lhs_blocks(col, y, h, J, cfg.use_moldiff, cfg, esc)
     if dense_lhs:
         # the reference's cost profile: a dense (ni*nz)^2 neg_symjac, then the
         # diagonal/diffusion updates and the column-loop band repack; the
@@ -514,13 +605,20 @@ def ros2_solver(net, col, st: State, cfg: Config, tables=None, dense_lhs=False, 
     k1_flat = solve(df)
     k1 = k1_flat.reshape(y.shape)
     yk2 = y + k1 / r
-    df = chemdf(net, yk2, M, k, tables).flatten() + diffdf(col, yk2, cfg.use_moldiff).flatten()
+    df = chemdf(net, yk2, M, k, tables).flatten() + diffdf(col, yk2, cfg.use_moldiff, cfg).flatten()
     rhs = df - 2. / (r * h) * k1_flat
     k2 = solve(rhs).reshape(y.shape)
     sol = y + 3. / (2. * r) * k1 + 1 / (2. * r) * k2
+    if cfg.use_fix_sp_bot:
+        # fixed bottom mixing ratios (op.py:1435-1436, 2945-2946)
+        fix_idx = [net.species.index(sp) for sp in cfg.use_fix_sp_bot.keys()]
+        fix_mix = np.array([cfg.use_fix_sp_bot[sp] for sp in cfg.use_fix_sp_bot.keys()])
+        sol[0, fix_idx] = fix_mix * col.M[0]
     delta = np.abs(sol - yk2)
     delta[ymix < cfg.mtol] = 0
     delta[sol < cfg.atol] = 0
+    if cfg.use_botflux or cfg.use_fix_sp_bot:
+        delta[0] = 0                     # op.py:2953
     pos = sol > 0
     delta = np.amax(delta[pos] / sol[pos])
     st.y = sol
@@ -627,6 +725,23 @@ def update_mu_dz(col, st: State):
     return col
 
 
+def esc_indices(species, cfg: Config):
+    """Species of diff_esc whose escape changes the step: through the top
+    flux of diffdf (use_topflux) or the _vm Jacobian (op.py:2101-2107)."""
+    if not cfg.diff_esc or not (cfg.use_topflux or (cfg.use_vm_mol and cfg.use_moldiff)):
+        return []
+    return [list(species).index(sp) for sp in cfg.diff_esc]
+
+
+def update_phi_esc(col, st: State, cfg: Config, species):
+    """Diffusion-limited escape flux at the top (op.py:986-999); mutates
+    col.top_flux."""
+    for i in esc_indices(species, cfg):
+        f = -col.Dzz[-1, i] * st.y[-1, i] * (1. / col.Hp[-1] - col.ms[i] * col.g[-1] / (Navo * kb * col.Tco[-1]))
+        col.top_flux[i] = max(f, cfg.max_flux * (-1))
+    return col
+
+
 def f_dy(st: State, cfg: Config):
     """op.py:1003-1015."""
     if st.count == 0:
@@ -692,8 +807,10 @@ def integrate(net, col, st: State, cfg: Config, max_steps=None, tables=None, upd
         st.y_prev = np.copy(st.y)                      # backup, op.py:938-942
         st.atom_loss_prev = st.atom_loss.copy()
         one_step(net, col, st, cfg, tables)
-        if update_mu and st.count % cfg.update_frq == 0:
-            update_mu_dz(col, st)
+        if st.count % cfg.update_frq == 0:
+            if update_mu:
+                update_mu_dz(col, st)
+            update_phi_esc(col, st, cfg, net.species)
         st.y = np.vstack(col.M) * st.ymix              # op.py:909-914
         f_dy(st, cfg)
         st.t += st.dt                                  # save_step, op.py:1087-1105

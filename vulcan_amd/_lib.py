@@ -26,7 +26,9 @@ class NetworkDesc(C.Structure):
 
 
 class Atm(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ('Tco', 'M', 'Kzz', 'Dzz', 'dzi', 'Ti', 'Hpi', 'g', 'Hp', 'tau')]
+    _fields_ = [(n, C.c_void_p) for n in ('Tco', 'M', 'Kzz', 'Dzz', 'dzi', 'Ti', 'Hpi', 'g', 'Hp', 'tau',
+                                          'vz', 'vm', 'top_flux', 'bot_flux', 'bot_vdep', 'fix_bot')] + \
+        [('use_topflux', C.c_int), ('use_botflux', C.c_int), ('esc', C.c_uint * 5), ('max_flux', C.c_double)]
 
 
 class Geom(C.Structure):
@@ -92,6 +94,7 @@ _SIGS = {
     'vk_photo_read': [VP, C.c_int, VP],
     'vk_integrator_set_photo': [VP, VP, C.c_int, C.c_int],
     'vk_update_mu_dz': [VP, VP, C.POINTER(Atm), C.POINTER(Geom), VP, VP, C.c_int, VP],
+    'vk_update_phi_esc': [VP, VP, C.POINTER(Atm), VP, VP],
     'vk_atom_sums': [VP, VP, VP, VP, VP],
     'vk_integrator_set_timing': [VP, C.c_int],
     'vk_integrator_set_graph': [VP, C.c_int],

@@ -48,6 +48,11 @@ class Column:
     Rp: float
     pref_indx: int
     tau: Optional[np.ndarray] = None   # (nz,) photolysis optical depth
+    # optional transport terms and boundary conditions (store.py:131-141)
+    vm: Optional[np.ndarray] = None        # (nz, ni) molecular-diffusion velocity (use_vm_mol)
+    top_flux: Optional[np.ndarray] = None  # (ni,) constant top flux, cm^-2 s^-1 (use_topflux, diff_esc)
+    bot_flux: Optional[np.ndarray] = None  # (ni,) bottom flux (use_botflux)
+    bot_vdep: Optional[np.ndarray] = None  # (ni,) deposition velocity, cm s^-1 (use_botflux)
 
     @property
     def nz(self) -> int:
@@ -87,6 +92,48 @@ def mol_diff_H2(Tco, n_0, species, mass, use_moldiff=True):
         mi = ms[i]
         Dzz[:, i] = 2.2965E17 * Tco_i ** 0.765 / n0_i * (16.04 / mi * (mi + 2.016) / 18.059) ** 0.5
     return Dzz, alpha, ms
+
+
+def mol_velocity_H2(Tco, n_0, ms, alpha, g, Hp, dz) -> np.ndarray:
+    """Molecular-diffusion velocity on the layers for use_vm_mol
+    (build_atm.py:731-740), with the layer-centred H2-background Dzz
+    (build_atm.py:666, 705).  The temperature step of the top layer wraps
+    round to the bottom layer (np.roll), as the reference's does."""
+    nz, ni = Tco.shape[0], ms.shape[0]
+    Dzz_cen = np.zeros((nz, ni))
+    for i in range(ni):
+        mi = ms[i]
+        Dzz_cen[:, i] = 2.2965E17 * Tco ** 0.765 / n_0 * (16.04 / mi * (mi + 2.016) / 18.059) ** 0.5
+    delta_T = np.roll(Tco, -1) - Tco
+    delta_T[0] = delta_T[1]
+    return -Dzz_cen * (ms[np.newaxis, :] * g[:, np.newaxis] / (Navo * kb * Tco[:, np.newaxis])
+                       - 1. / Hp[:, np.newaxis] + alpha / Tco[:, np.newaxis] * (delta_T[:, np.newaxis]) / dz[:, np.newaxis])
+
+
+def read_bc_flux(species, top_file=None, bot_file=None):
+    """Boundary-condition files (build_atm.py:743-773): the top file has
+    lines ``species flux``, the bottom file ``species flux v_dep [fix_mix]``
+    ('#' starts a comment).  Returns (top_flux, bot_flux, bot_vdep,
+    bot_fix_sp), each (ni,), zero where a file names no value."""
+    ni = len(species)
+    index = {s: i for i, s in enumerate(species)}
+    top, bot, vdep, fix = np.zeros(ni), np.zeros(ni), np.zeros(ni), np.zeros(ni)
+
+    def rows(path):
+        with open(path) as f:
+            for line in f:
+                if not line.startswith('#') and line.strip():
+                    yield line.split()
+    if top_file:
+        for li in rows(top_file):
+            top[index[li[0]]] = float(li[1])
+    if bot_file:
+        for li in rows(bot_file):
+            bot[index[li[0]]] = float(li[1])
+            vdep[index[li[0]]] = float(li[2])
+            if len(li) > 3:
+                fix[index[li[0]]] = float(li[3])
+    return top, bot, vdep, fix
 
 
 def thermal_alpha(species, mass) -> np.ndarray:

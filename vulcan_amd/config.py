@@ -1,10 +1,10 @@
 """Numerical knobs, with VULCAN's names and defaults (vulcan_cfg.py:124-159).
 
 ``Config`` is a plain attribute bag so that host code reads the same names the
-reference reads from ``vulcan_cfg``.  Condensation, ions, fixed-species and flux
-boundary conditions are outside the accelerated path; their switches exist
-only so that a configuration asking for them is rejected loudly instead of
-being silently ignored.
+reference reads from ``vulcan_cfg``.  Condensation, settling and ions are
+outside the accelerated path; their switches exist only so that a
+configuration asking for them is rejected loudly instead of being silently
+ignored.
 """
 from __future__ import annotations
 
@@ -31,7 +31,9 @@ class Config:
     use_ion: bool = False
     use_topflux: bool = False
     use_botflux: bool = False
-    use_fix_sp_bot: Dict[str, float] = dataclasses.field(default_factory=dict)
+    use_fix_sp_bot: Dict[str, float] = dataclasses.field(default_factory=dict)   # vulcan_cfg.py:100
+    diff_esc: List[str] = dataclasses.field(default_factory=lambda: ['H'])      # vulcan_cfg.py:101
+    max_flux: float = 1e13                                                      # vulcan_cfg.py:102
     atm_base: str = 'H2'
     rocky: bool = False
     gs: float = 2140.0                # vulcan_cfg.py:84
@@ -78,11 +80,8 @@ class Config:
 
     def check_supported(self) -> None:
         """Refuse switches whose physics is not on the accelerated path."""
-        bad = [name for name in ('use_vm_mol', 'use_settling', 'use_condense', 'use_ion',
-                                 'use_topflux', 'use_botflux', 'use_vz')
+        bad = [name for name in ('use_settling', 'use_condense', 'use_ion')
                if getattr(self, name)]
-        if self.use_fix_sp_bot:
-            bad.append('use_fix_sp_bot')
         if bad:
             raise NotImplementedError(
                 'configuration switches not supported by the device Ros2 path: '

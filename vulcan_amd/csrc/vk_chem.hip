@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(BLK) k_rhs(
             v[p] = pair_rate((const int4 *)reac, (const int4 *)prod, kl, yl, m, p);
         __syncthreads();
     }
-    ColAtm ca = col_atm(atm, x.off, x.nz, ni);
+    ColAtm ca = col_atm(atm, x.c, x.off, x.nz, ni);
     double c2 = 0.0;
     if (k1) c2 = 2. / (R_ROS * dtc[x.c]);
     for (int i = threadIdx.x; i < ni; i += blockDim.x) {
@@ -244,11 +244,24 @@ __global__ void __launch_bounds__(BLK) k_rhs(
 // D = c0 I - J - J_diff, Lsub, Usup), one workgroup per layer
 // ---------------------------------------------------------------------------
 
+// The diagonal of a layer's lhs block after c0 - J: the diffusion terms in
+// the reference's order.  xmode 1: the escape term first (top layer, _vm
+// operators, op.py:2101-2107); xmode 2: the deposition velocity last (bottom
+// layer, op.py:2029).
+__device__ __forceinline__ double lhs_diag(double val, int i, int mol, const double *dg0, const double *dg1,
+                                           const double *dgx, int xmode) {
+    if (xmode == 1) val = val - dgx[i];
+    val = val - dg0[i];
+    if (mol) val = val - dg1[i];
+    if (xmode == 2) val = val - dgx[i];
+    return val;
+}
+
 // one entry of the lhs (mode 1) or of the Jacobian (mode 0): its terms in order
 __device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_ent,
                                           const vk_jac_term *__restrict__ jac_term, const double *dv, int ni,
                                           int mode, int mol, double c0, const double *dg0, const double *dg1,
-                                          double *__restrict__ Dl) {
+                                          const double *dgx, int xmode, double *__restrict__ Dl) {
     const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
     double J = 0.0;
     if (en.z > en.y) {
@@ -272,11 +285,7 @@ __device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_en
     }
     const int i = en.w;
     double val = -J;
-    if (en.x == i * ni + i) {
-        val = c0 + val;
-        val = val - dg0[i];
-        if (mol) val = val - dg1[i];
-    }
+    if (en.x == i * ni + i) val = lhs_diag(c0 + val, i, mol, dg0, dg1, dgx, xmode);
     Dl[en.x] = val;
 }
 
@@ -293,7 +302,8 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     double *yl = smem;                       // ni
     double *dg0 = smem + VK_NI_MAX;          // diffusion terms of the diagonal, per species
     double *dg1 = smem + 2 * VK_NI_MAX;
-    double *dv = smem + 3 * VK_NI_MAX;       // nslot
+    double *dgx = smem + 3 * VK_NI_MAX;      // escape (top) or deposition (bottom) term
+    double *dv = smem + 4 * VK_NI_MAX;       // nslot
     const int l = blockIdx.x;
     LayerCtx x = layer_ctx(off, layer_col, l);
     if (st && st[x.c].done) return;
@@ -311,16 +321,28 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
     }
     double c0 = 0.0;
+    int xmode = 0;
     if (mode == 1) {
         // diffusion part of the lhs (op.py:2012-2040): once per species
         c0 = 1. / (R_ROS * dtc[x.c]);
         double ysm, ys0, ysp;
         ysum_around(ysum, x, ysm, ys0, ysp);
-        ColAtm ca = col_atm(atm, x.off, x.nz, ni);
+        ColAtm ca = col_atm(atm, x.c, x.off, x.nz, ni);
+        if (x.j == 0 && ca.use_bot) xmode = 2;
+        else if (x.j == x.nz - 1 && ca.vm && mol && atm.top_flux && esc_any(atm)) xmode = 1;
         for (int i = threadIdx.x; i < ni; i += blockDim.x) {
             LhsDiff r = lhs_diff(ca, x.j, i, ysm, ys0, ysp, ms[i], alpha[i], mol != 0);
             dg0[i] = r.d0;
             dg1[i] = r.d1;
+            if (xmode == 2) {
+                dgx[i] = -1. * ca.bot_vdep[i] / ca.dzi[0];
+            } else if (xmode == 1) {
+                // diff_lim = top_flux / y for escaping species with y > 0 (no 1/dzi, as there)
+                double e = 0.0;
+                if (((atm.esc[i >> 5] >> (i & 31)) & 1u) && yrow[i] > 0)
+                    e = 0.0 + atm.top_flux[(size_t)x.c * ni + i] / yrow[i];
+                dgx[i] = e;
+            }
             double u = 0.0, lo = 0.0;
             if (r.has_u) { u = 0.0 - r.u0; if (r.has_mol) u = u - r.u1; }
             if (r.has_l) { lo = 0.0 - r.l0; if (r.has_mol) lo = lo - r.l1; }
@@ -338,7 +360,8 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     // entries of a layer are split over the grid's y dimension (each
     // workgroup forms the layer's shared terms itself).
     const int e0 = threadIdx.x + blockIdx.y * blockDim.x, es = blockDim.x * gridDim.y;
-    for (int q = e0; q < nheavy; q += es) jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0, dg1, Dl);
+    for (int q = e0; q < nheavy; q += es)
+        jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
 #pragma unroll 2
     for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
@@ -362,11 +385,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
             } else {
                 const int i = e / ni;
                 val = -J;
-                if (e == i * ni + i) {
-                    val = c0 + val;
-                    val = val - dg0[i];
-                    if (mol) val = val - dg1[i];
-                }
+                if (e == i * ni + i) val = lhs_diag(c0 + val, i, mol, dg0, dg1, dgx, xmode);
             }
         }
         Dl[e] = val;
@@ -382,7 +401,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
 static size_t rhs_smem(const vk_plan *p) { return sizeof(double) * (VK_NI_MAX + (size_t)p->npairs); }
-static size_t jac_smem(const vk_plan *p) { return sizeof(double) * (3 * VK_NI_MAX + (size_t)p->nslot); }
+static size_t jac_smem(const vk_plan *p) { return sizeof(double) * (4 * VK_NI_MAX + (size_t)p->nslot); }
 
 extern "C" int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, double *d_k, void *stream) {
     if (!plan || !cols || !atm || !d_k) return VK_EINVAL;

@@ -68,14 +68,18 @@ __global__ void k_yk2(int n, int ni, const int *__restrict__ layer_col, const do
     yk2[idx] = y[idx] + k1[idx] / R_ROS;
 }
 
-// sol, delta partial, ymix; with clip also the clip/loss partials (op.py:2932-2993, 2447-2487)
+// sol, delta partial, ymix; with clip also the clip/loss partials (op.py:2932-2993, 2447-2487).
+// fix_bot (may be NULL): fixed bottom mixing ratios, < 0 free (op.py:2945-2946);
+// mask_bot: layer 0 leaves delta (op.py:2953)
 __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *__restrict__ off,
                                                   const int *__restrict__ layer_col,
                                                   const double *__restrict__ atom_mat, const double *y,
                                                   const double *ymix_in, const double *__restrict__ k1,
                                                   const double *__restrict__ k2, const double *__restrict__ yk2,
                                                   vk_params prm, int clip, const vk_colstate *__restrict__ st,
-                                                  double *ynew, double *ymixnew, double *__restrict__ part) {
+                                                  const double *__restrict__ fix_bot, const double *__restrict__ M,
+                                                  int mask_bot, double *ynew, double *ymixnew,
+                                                  double *__restrict__ part) {
     __shared__ double sol[VK_NI_MAX];
     __shared__ double red[256];
     __shared__ int negf[256];
@@ -86,12 +90,16 @@ __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *
     layer_of(off, layer_col, l, c, o, nz);
     if (st[c].done) return;
     const size_t base = (size_t)l * ni;
+    const bool bottom = l == o;
+    const double *fb = fix_bot && bottom ? fix_bot + (size_t)c * ni : nullptr;
     double rmax = -1.0;
     for (int i = threadIdx.x; i < ni; i += blockDim.x) {
         double s = y[base + i] + C3 * k1[base + i] + C1 * k2[base + i];
+        if (fb && fb[i] >= 0) s = fb[i] * M[l];
         double d = fabs(s - yk2[base + i]);
         if (ymix_in[base + i] < prm.mtol) d = 0;
         if (s < prm.atol) d = 0;
+        if (bottom && mask_bot) d = 0;
         if (s > 0) rmax = fmax(rmax, d / s);
         sol[i] = s;
     }
@@ -257,7 +265,7 @@ __global__ void __launch_bounds__(DECIDE_NT) k_decide(int natom, const int *__re
     }
     if (f) {
         for (int a = 0; a < natom; a++) atom_loss_prev[(size_t)c * natom + a] = atom_loss[(size_t)c * natom + a];
-        if (prm.use_update_mu && s.count % prm.update_frq == 0) f |= 16;
+        if (s.count % prm.update_frq == 0) f |= 16;      // update_mu_dz / update_phi_esc due
         // save_step (op.py:1087-1105)
         s.t += s.dt;
         s.count += 1;
@@ -322,6 +330,25 @@ __global__ void k_update_mu(int ni, const int *__restrict__ off, const double *_
     for (int j = threadIdx.x; j < nz - 1; j += blockDim.x) {
         atm.dzi[o + j] = 0.5 * (geo.dz[o + j + 1] + geo.dz[o + j]);
         atm.Hpi[o + j] = 0.5 * (atm.Hp[o + j] + atm.Hp[o + j + 1]);
+    }
+}
+
+// update_phi_esc (op.py:986-999) for flagged columns: the diffusion-limited
+// top flux of each escaping species from the step's y (before the
+// renormalisation), capped at max_flux
+__global__ void k_phi_esc(int ni, const int *__restrict__ off, const double *__restrict__ ms,
+                          const double *__restrict__ y, vk_atm atm, const int *__restrict__ flags, int force,
+                          const vk_colstate *__restrict__ st) {
+    const int c = blockIdx.x;
+    if (force) { if (st && st[c].done) return; }
+    else if (!(flags[c] & 16)) return;
+    const int t = off[c + 1] - 1;      // top layer; Dzz[-1] is interface t - 1
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        if (!((atm.esc[i >> 5] >> (i & 31)) & 1u)) continue;
+        const double f = -atm.Dzz[(size_t)(t - 1) * ni + i] * y[(size_t)t * ni + i] *
+                         (1. / atm.Hp[t] - ms[i] * atm.g[t] / (NAVO_KB * atm.Tco[t]));
+        const double cap = atm.max_flux * (-1);
+        atm.top_flux[(size_t)c * ni + i] = cap > f ? cap : f;      // Python's max(f, cap)
     }
 }
 
@@ -448,6 +475,10 @@ __global__ void __launch_bounds__(256) k_conv(int ni, const int *__restrict__ of
 extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
     if (!n || !out || n->ni <= 0 || n->npairs <= 0) return VK_EINVAL;
     if (n->ni > VK_NI_MAX || n->natom > VK_NATOM_MAX) return VK_ETOOBIG;
+    // the per-layer kernels keep a layer's pair rates / Jacobian slots in LDS (64 KiB)
+    if (sizeof(double) * ((size_t)VK_NI_MAX + n->npairs) > 65536 ||
+        sizeof(double) * (4 * (size_t)VK_NI_MAX + n->nslot) > 65536)
+        return VK_ETOOBIG;
     vk_plan *p = new vk_plan();
     p->ni = n->ni; p->npairs = n->npairs; p->nr = 2 * n->npairs; p->natom = n->natom;
     p->factor_variant = VK_FACTOR_REGISTER;
@@ -737,6 +768,13 @@ extern "C" int vk_integrator_stats(const vk_integrator *it, double *factor_ms, l
 }
 
 // one Ros2 attempt on all active columns (op.py:2890-2932): k1 in it->x1, k2 in it->x2
+// escape is active when a species is marked and there is a top-flux array
+static bool esc_on(const vk_atm *a) {
+    unsigned m = 0;
+    for (int w = 0; w < VK_SPECIES_WORDS; w++) m |= a->esc[w];
+    return m != 0 && a->top_flux;
+}
+
 static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *prm, const double *k, double *y,
                         double *ymix, vk_colstate *st, int clip, bool timed, hipStream_t s) {
     const vk_plan *p = it->plan;
@@ -766,8 +804,10 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b2, it->x2, it->work, it->ztmp, prm->refine,
                                 st, &it->bcr, s);   // k2
     if (r) return r;
+    const int mask_bot = (atm->use_botflux && atm->bot_flux && atm->bot_vdep) || atm->fix_bot;
     hipLaunchKernelGGL(k_finalize, dim3(cl->L), dim3(256), 0, s, ni, p->natom, cl->d_off, cl->d_layer_col,
-                       p->atom_mat, y, ymix, it->x1, it->x2, it->yk2, *prm, clip, st, y, ymix, it->part);
+                       p->atom_mat, y, ymix, it->x1, it->x2, it->yk2, *prm, clip, st, atm->fix_bot, atm->M, mask_bot,
+                       y, ymix, it->part);
     VK_CHECK(hipGetLastError());
     return 0;
 }
@@ -819,6 +859,9 @@ static int loop_body(vk_integrator *it, const vk_atm *atm, const vk_geom *g, con
     if (prm->use_update_mu)
         hipLaunchKernelGGL(k_update_mu, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, p->ms, d_ymix, *atm,
                            *g, it->flags, 0, d_st);
+    if (esc_on(atm))
+        hipLaunchKernelGGL(k_phi_esc, dim3(cl->ncols), dim3(64), 0, s, p->ni, cl->d_off, p->ms, d_y, *atm, it->flags,
+                           0, d_st);
     hipLaunchKernelGGL(k_apply, dim3(cl->L), dim3(128), 0, s, p->ni, cl->d_layer_col, it->flags, atm->M, d_ymix, d_y,
                        d_y_prev, d_st, it->yhist, it->history, (size_t)cl->L * p->ni);
     hipLaunchKernelGGL(k_conv, dim3(cl->ncols), dim3(256), 0, s, p->ni, cl->d_off, *prm, *atm, d_y, d_ymix,
@@ -948,6 +991,16 @@ extern "C" int vk_integrator_set_photo(vk_integrator *it, vk_photo *photo, int i
 extern "C" int vk_integrator_set_graph(vk_integrator *it, int on) {
     if (!it) return VK_EINVAL;
     it->use_graph = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int vk_update_phi_esc(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
+                                 void *stream) {
+    if (!plan || !cols || !atm || !d_y || !atm->Dzz || !atm->Hp || !atm->g || !atm->Tco) return VK_EINVAL;
+    if (!esc_on(atm)) return 0;
+    hipLaunchKernelGGL(k_phi_esc, dim3(cols->ncols), dim3(64), 0, (hipStream_t)stream, plan->ni, cols->d_off, plan->ms,
+                       d_y, *atm, nullptr, 1, nullptr);
+    VK_CHECK(hipGetLastError());
     return 0;
 }
 

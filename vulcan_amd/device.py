@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .atmosphere import Column, thermal_alpha
+from .atmosphere import Column, mol_velocity_H2, thermal_alpha
 from .config import Config
 from .network import Network, NetworkTables, build_tables
 
@@ -126,15 +126,27 @@ class DeviceColumns:
             pass
 
 
-def conv_ignore_mask(cfg: Config, species: Sequence[str]) -> np.ndarray:
-    """Bit mask of ``cfg.conver_ignore`` over the network's species (bit
-    i % 32 of word i // 32); an unknown name raises ValueError, as the
-    reference's ``species.index(sp)`` does (op.py:1045-1046)."""
+def species_mask(names: Sequence[str], species: Sequence[str]) -> np.ndarray:
+    """Bit mask of species names (bit i % 32 of word i // 32); an unknown
+    name raises ValueError, as the reference's ``species.index(sp)`` does."""
     mask = np.zeros(5, np.uint32)
-    for sp in cfg.conver_ignore:
+    for sp in names:
         i = list(species).index(sp)
         mask[i // 32] |= np.uint32(1 << (i % 32))
     return mask
+
+
+def conv_ignore_mask(cfg: Config, species: Sequence[str]) -> np.ndarray:
+    """``cfg.conver_ignore`` as a species mask (op.py:1045-1046)."""
+    return species_mask(cfg.conver_ignore, species)
+
+
+def escape_active(cfg: Config) -> bool:
+    """Whether diffusion-limited escape (diff_esc, op.py:986-999) changes the
+    step: through the top flux of diffdf (use_topflux, op.py:1589-1592) or
+    the Jacobian of the _vm operators (op.py:2101-2107).  Otherwise the
+    reference's update of top_flux has no effect and the device skips it."""
+    return bool(cfg.diff_esc) and bool(cfg.use_topflux or (cfg.use_vm_mol and cfg.use_moldiff))
 
 
 def params_from(cfg: Config, use_conv: bool = True, use_update_mu: bool = True,
@@ -194,6 +206,30 @@ class ColumnBatch:
         self.Ti, self.Hpi = T(stack_iface('Ti')), T(stack_iface('Hpi'))
         self.Dzz = T(stack_iface('Dzz', ni))
         self.tau = T(stack_layers('tau')) if all(c.tau is not None for c in cols) else None
+        # transport terms and boundary conditions (store.py:131-141); an
+        # array stays unallocated (NULL on the device) while its term is off
+        self.vz = T(stack_iface('vz')) if any(np.any(c.vz) for c in cols) else None
+        self.vm = None
+        if cfg.use_vm_mol and cfg.use_moldiff:
+            # build_atm.py:735-736 from the set-up geometry when the column has none
+            self.vm = T(np.concatenate([c.vm if c.vm is not None else
+                                        mol_velocity_H2(c.Tco, c.M, c.ms, c.alpha, c.g, c.Hp, c.dz)
+                                        for c in cols]))
+
+        def per_column(name):
+            return T(np.stack([np.zeros(ni) if getattr(c, name) is None else getattr(c, name) for c in cols]))
+        esc = escape_active(cfg)
+        self.top_flux = per_column('top_flux') if (cfg.use_topflux or esc) else None
+        self.bot_flux = per_column('bot_flux') if cfg.use_botflux else None
+        self.bot_vdep = per_column('bot_vdep') if cfg.use_botflux else None
+        self.fix_bot = None
+        if cfg.use_fix_sp_bot:
+            fb = np.full(ni, -1.0)
+            for name, mix in cfg.use_fix_sp_bot.items():
+                if not mix >= 0:
+                    raise ValueError('use_fix_sp_bot[%r] = %r is not a mixing ratio' % (name, mix))
+                fb[list(net.species).index(name)] = float(mix)
+            self.fix_bot = T(np.tile(fb, (nc, 1)))
         self.dz = T(stack_layers('dz'))
         self.mu = T(stack_layers('mu'))
         self.zco = T(np.concatenate([c.zco for c in cols]))
@@ -228,7 +264,14 @@ class ColumnBatch:
         self.set_graph(graph)
         self.atm = _lib.Atm(Tco=dptr(self.Tco), M=dptr(self.M), Kzz=dptr(self.Kzz), Dzz=dptr(self.Dzz),
                             dzi=dptr(self.dzi), Ti=dptr(self.Ti), Hpi=dptr(self.Hpi), g=dptr(self.g),
-                            Hp=dptr(self.Hp), tau=dptr(self.tau))
+                            Hp=dptr(self.Hp), tau=dptr(self.tau), vz=dptr(self.vz), vm=dptr(self.vm),
+                            top_flux=dptr(self.top_flux), bot_flux=dptr(self.bot_flux),
+                            bot_vdep=dptr(self.bot_vdep), fix_bot=dptr(self.fix_bot),
+                            use_topflux=int(cfg.use_topflux), use_botflux=int(cfg.use_botflux),
+                            max_flux=float(cfg.max_flux))
+        if esc:
+            for w, m in enumerate(species_mask(cfg.diff_esc, net.species)):
+                self.atm.esc[w] = int(m)
         self.geom = _lib.Geom(lnp=dptr(self.lnp), zco=dptr(self.zco), dz=dptr(self.dz), mu=dptr(self.mu),
                               gs=dptr(self.gs), Rp=dptr(self.Rp), pref=dptr(self.pref))
         if k is None:
@@ -242,6 +285,11 @@ class ColumnBatch:
     def compute_rates(self):
         _lib.check(self._lib.vk_rates(self.plan.handle, self.layout.handle, C.byref(self.atm), dptr(self.k),
                                       stream_handle()), 'vk_rates')
+
+    def update_phi_esc(self):
+        """update_phi_esc (op.py:986-999) now, from the current y."""
+        _lib.check(self._lib.vk_update_phi_esc(self.plan.handle, self.layout.handle, C.byref(self.atm),
+                                               dptr(self.y), stream_handle()), 'vk_update_phi_esc')
 
     def reset_element_baseline(self):
         """atom_ini from the current y (build_atm.py:283-291); losses to zero."""

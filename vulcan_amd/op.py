@@ -18,8 +18,12 @@ argument meaning on the same ``var``/``atm``/``para`` attribute names
 
 ``compute_tau`` / ``compute_flux`` / ``compute_J`` run the photochemical
 radiative transfer on the device from the tables VULCAN's set-up left in
-``var``.  Condensation, ions and flux boundary conditions are not part of
-the accelerated path and raise ``NotImplementedError``.
+``var``.  Vertical advection (``atm.vz``), the ``_vm`` operators
+(``atm.vm``), top/bottom fluxes (``atm.top_flux``, ``bot_flux``,
+``bot_vdep``), fixed bottom mixing ratios and diffusion-limited escape
+follow the same ``vulcan_cfg`` switches as the reference.  Condensation,
+settling, ions and photoionisation are not part of the accelerated path
+and raise ``NotImplementedError``.
 Numerical failure is never raised (negative y, delta and element loss are
 reported through ``var``/``para``, as in the reference).
 """
@@ -43,7 +47,24 @@ def _column_from_atm(atm, nz: int, ni: int) -> Column:
                   dz=g('dz', np.zeros(nz)), zco=g('zco', np.zeros(nz + 1)), dzi=g('dzi'),
                   Ti=g('Ti', np.zeros(nz - 1)), Hpi=g('Hpi', np.zeros(nz - 1)), gs=float(getattr(atm, 'gs', 0.0)),
                   Rp=float(getattr(atm, 'Rp', 1.0)), pref_indx=int(getattr(atm, 'pref_indx', 0)),
-                  tau=g('tau', None))
+                  tau=g('tau', None), vm=g('vm', None), top_flux=g('top_flux', None), bot_flux=g('bot_flux', None),
+                  bot_vdep=g('bot_vdep', None))
+
+
+def _sync_bc(b, col: Column, T, pad):
+    """Copy the column's transport and boundary arrays into a batch built
+    for it (the reference's update_phi_esc rewrites atm.top_flux between
+    solver calls)."""
+    if b.vz is not None:
+        b.vz.copy_(T(pad(col.vz)))
+    elif np.any(col.vz):
+        raise ValueError('atm.vz became nonzero after the device batch was built without advection')
+    if b.vm is not None and col.vm is not None:
+        b.vm.copy_(T(col.vm))
+    for name in ('top_flux', 'bot_flux', 'bot_vdep'):
+        dst, src = getattr(b, name), getattr(col, name)
+        if dst is not None and src is not None:
+            dst.copy_(T(np.asarray(src, dtype=np.float64)[None]))
 
 
 class Ros2:
@@ -58,6 +79,12 @@ class Ros2:
                 # the reference fails the same way, at its first conv() test
                 # (species.index(sp), op.py:1045-1046)
                 raise ValueError('conver_ignore names %r, which is not a species of the network' % (sp,))
+        from .device import escape_active
+        if escape_active(self.cfg):
+            for sp in self.cfg.diff_esc:
+                if sp not in net.species:
+                    # species.index(sp) in update_phi_esc (op.py:989-993)
+                    raise ValueError('diff_esc names %r, which is not a species of the network' % (sp,))
         self.chem = ChemFuns(net)
         self.atom_list = net.atoms()
         self._atom_mat = net.atom_matrix()
@@ -148,6 +175,7 @@ class Ros2:
         Dz = np.zeros((nz, self.net.ni))
         Dz[:nz - 1] = col.Dzz
         b.Dzz.copy_(T(Dz))
+        _sync_bc(b, col, T, pad)
         return b
 
     def solver(self, var, atm, para):
@@ -346,6 +374,8 @@ class Integration:
             setattr(atm, name, b.column(name, 0))
         for name in ('dzi', 'Hpi'):
             setattr(atm, name, b.column(name, 0)[:nz - 1])
+        if b.top_flux is not None:
+            atm.top_flux = b.top_flux.cpu().numpy()[0].copy()     # update_phi_esc's result
         loss = b.atom_loss.cpu().numpy()[0]
         if getattr(var, 'atom_loss', None) is None:
             var.atom_loss = {}
