@@ -55,21 +55,23 @@ __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
 }
 
 // Maximum over the 64 lanes (all active), wave-uniform: DPP max within rows
-// of 16 (quad swaps, half-row and row mirrors), then the four row maxima
-// through readlane.
+// of 16 (quad swaps, half-row and row mirrors), then row_bcast:15 and
+// row_bcast:31 carry the row maxima up to lane 63, read once.
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false));
     v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false));
-    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0), b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
-    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32), d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
-    return max(max(a, b), max(c, d));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x142, 0xa, 0xf, false));   // rows 1, 3 += row 0, 2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x143, 0xc, 0xf, false));   // rows 2, 3 += row 1
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Pivot steps p0 .. p0+kend-1 on the panel pan[k][row] (one wave).  The
 // pivot key is the upper word of |value| with the row in its low byte: the
-// largest magnitude to 12 mantissa bits, ties to the lowest row.
+// largest magnitude to 12 mantissa bits, ties to the lowest row.  The pivot
+// bookkeeping (ipos of the chosen rows, perm of the steps) stays in
+// registers until the panel ends: no LDS write waits inside a step.
 template <int NB>
 __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * NB + 1], int ni, int p0, int kend,
                                           int tag) {
@@ -77,14 +79,16 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
     constexpr int RPL = (N16 + 63) / 64;
     const int l = threadIdx.x & 63;
     double v[RPL][16];
-    bool av[RPL];
+    // this lane's row q: -1 still available, -2 not a candidate (padding or
+    // chosen in an earlier panel), >= 0 the pivot step that chose it
+    int code[RPL];
     bool zero = false;
 #pragma unroll
     for (int q = 0; q < RPL; q++) {
         const int r = l + 64 * q;
 #pragma unroll
         for (int k = 0; k < 16; k++) v[q][k] = (r < N16) ? pan[k][r] : 0.0;
-        av[q] = (r < ni) && sh.ipos[r] < 0;
+        code[q] = (r < ni) && sh.ipos[r] < 0 ? -1 : -2;
     }
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
@@ -95,7 +99,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
             const int r = l + 64 * q;
             const unsigned key =
                 ((unsigned)(__double_as_longlong(fabs(v[q][kk])) >> 32) & ~0xffu) | (unsigned)(255 - r);
-            best = av[q] && key > best ? key : best;
+            best = code[q] == -1 && key > best ? key : best;
         }
         best = wave_max_u32(best);
         const int p = best ? 255 - (int)(best & 0xffu) : 0;
@@ -131,15 +135,15 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
 #pragma unroll
                 for (int k = 0; k < 16; k++) v[q][k] = rs[k];
                 v[q][kk] = inv;
-                av[q] = false;
+                code[q] = p0 + kk;
             }
-        if (l == pl) sh.ipos[p] = p0 + kk;
         if (l == 0) sh.perm[p0 + kk] = p;
     }
     if (zero && l == 0 && sh.info == 0) sh.info = tag;
 #pragma unroll
     for (int q = 0; q < RPL; q++) {
         const int r = l + 64 * q;
+        if (code[q] >= 0) sh.ipos[r] = code[q];
         if (r < N16) {
 #pragma unroll
             for (int k = 0; k < 16; k++) pan[k][r] = v[q][k];
