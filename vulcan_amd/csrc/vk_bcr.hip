@@ -31,7 +31,7 @@ namespace {
 // invert A[layer] for the listed layers (the eliminated ones of a level, or
 // the top layer of each column) into Dinv[layer] (transposed storage)
 template <int NB>
-__global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 *__restrict__ el,
+__global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : 2) k_bcr_invert(int ni, int n, const int4 *__restrict__ el,
                                                        const int2 *__restrict__ top, const int *__restrict__ off,
                                                        const double *__restrict__ A, double *__restrict__ Dinv,
                                                        int *__restrict__ info, const vk_colstate *__restrict__ st) {
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(NT, 2) k_bcr_invert(int ni, int n, const int4 
     __shared__ BlkShared<NB> sh;
     if (threadIdx.x == 0) sh.info = 0;
     const size_t nn = (size_t)ni * ni;
-    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    f64x4 acc[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC];
     blk_load_schur<NB>(acc, ni, A + layer * nn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     blk_invert<NB>(acc, ni, Dinv + layer * nn, sh, layer - off[c] + 1);
     if (threadIdx.x == 0 && info && sh.info) atomicCAS(&info[c], 0, sh.info);
@@ -449,7 +449,7 @@ int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const d
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
         // level 0 inverts its eliminated blocks from D (the level-0 update writes A)
         if (ne > 0)
-            hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(ne), dim3(NT), 0, s, ni, ne, b.d_elim + e0,
+            hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(ne), dim3(blk_nt(NB)), 0, s, ni, ne, b.d_elim + e0,
                                (const int2 *)nullptr, cols->d_off, lev == 0 ? D : ws->A, Dinv, info, st);
         if (ns > 0) {
             if (lev == 0)
@@ -464,7 +464,7 @@ int launch_bcr_factor_n(int ni, const vk_columns *cols, const double *D, const d
                                                              b.d_elim + e0, Dinv, ws, st, s);
         }
     }
-    hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(b.ntop), dim3(NT), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top,
+    hipLaunchKernelGGL((k_bcr_invert<NB>), dim3(b.ntop), dim3(blk_nt(NB)), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top,
                        cols->d_off, ws->A, Dinv, info, st);
     VK_CHECK(hipGetLastError());
     return 0;

@@ -8,6 +8,16 @@ namespace {
 
 constexpr int NT = 256;
 
+// Threads of a blocked-inversion workgroup: 4 waves up to 112 rows, 8 waves
+// above (the accumulator tiles of 128-160 rows would not fit 4 waves'
+// registers), with the waves in a 2 x (waves / 2) grid over the tiles.
+constexpr int blk_nt(int nb) { return nb >= 7 ? 512 : 256; }
+template <int NB>
+struct BlkGrid {
+    static constexpr int NTH = blk_nt(NB), NW = NTH / 64, WC = NW / 2;
+    static constexpr int TBR = (NB + 1) / 2, TBC = (NB + WC - 1) / WC;   // tiles per wave: rows, columns
+};
+
 // ---------------------------------------------------------------------------
 // Blocked Gauss-Jordan inversion on FP64 MFMA (factor variant 1)
 //
@@ -24,8 +34,9 @@ constexpr int NT = 256;
 // (P: the panel's pivot rows p_s, Pan: the factored panel columns), which is
 // a 16-deep GEMM and runs on v_mfma_f64_16x16x4f64.  Three barriers per
 // panel.  The matrix lives in the MFMA
-// accumulator layout: 16 x 16 tiles, wave (wr, wc) of the 2 x 2 wave grid
-// owns row blocks wr, wr+2, ... and column blocks wc, wc+2, ...; lane l,
+// accumulator layout: 16 x 16 tiles, wave (wr, wc) of the 2 x WC wave grid
+// (WC = 2 or 4, BlkGrid) owns row blocks wr, wr+2, ... and column blocks
+// wc, wc+WC, ...; lane l,
 // register r of a tile holds element (4*r + l/16, l%16) (operands: A(i, k)
 // in lane i + 16 k, B(k, j) in lane j + 16 k; checked by
 // vk_selftest_mfma_f64).
@@ -38,13 +49,17 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 template <int NB>
 struct BlkShared {
     static constexpr int N16 = 16 * NB;
+    // above 128 rows the panel wave parks its accumulator tiles here during
+    // the panel (its registers then hold the panel alone)
+    static constexpr bool PARK = NB >= 9;
     double pan[2][16][N16 + 1];   // panel columns, [k][row], double-buffered by panel parity
     double rb[16][N16];           // old pivot rows of the panel, [s][col]
-    double dump[NT];              // per-thread target of discarded writes
+    double dump[blk_nt(NB)];      // per-thread target of discarded writes
     double prow[16];              // current pivot row of the panel factorisation
     int ipos[N16];
     int perm[N16];
     int info;
+    double park[PARK ? 4 * ((NB + 1) / 2) * ((NB + 3) / 4) : 1][64];
 };
 
 template <int CTRL>
@@ -158,13 +173,14 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
 // transposed to X (X[m][i] = inv[i][m]).  A zero pivot records `tag` in
 // sh.info (sticky).
 template <int NB>
-__device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni, double *__restrict__ X,
-                                           BlkShared<NB> &sh, int tag) {
-    constexpr int N16 = 16 * NB, TB = (NB + 1) / 2;
+__device__ __forceinline__ void blk_invert(f64x4 (&acc)[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC], int ni,
+                                           double *__restrict__ X, BlkShared<NB> &sh, int tag) {
+    using G = BlkGrid<NB>;
+    constexpr int N16 = 16 * NB, TBR = G::TBR, TBC = G::TBC, WC = G::WC;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const int wr = w >> 1, wc = w & 1;
+    const int wr = w / WC, wc = w % WC;
     const int lg = l >> 4, lc = l & 15;
-    for (int t = tid; t < N16; t += NT) sh.ipos[t] = -1;
+    for (int t = tid; t < N16; t += G::NTH) sh.ipos[t] = -1;
     __syncthreads();
     const int npan = (ni + 15) >> 4;
     for (int pb = 0; pb < npan; pb++) {
@@ -172,10 +188,10 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
         const int p0 = 16 * pb, kend = min(16, ni - p0);
         // the owners of column block pb publish the panel
 #pragma unroll
-        for (int tj = 0; tj < TB; tj++) {
-            if (wc + 2 * tj != pb) continue;   // static register index, uniform test
+        for (int tj = 0; tj < TBC; tj++) {
+            if (wc + WC * tj != pb) continue;   // static register index, uniform test
 #pragma unroll
-            for (int ti = 0; ti < TB; ti++) {
+            for (int ti = 0; ti < TBR; ti++) {
                 const int bi = wr + 2 * ti;
                 if (bi < NB) {
 #pragma unroll
@@ -184,20 +200,38 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
             }
         }
         __syncthreads();
-        if (w == ((pb + (int)blockIdx.x) & 3)) blk_panel<NB>(sh, pan, ni, p0, kend, tag);
+        if (w == ((pb + (int)blockIdx.x) & (G::NW - 1))) {
+            if constexpr (BlkShared<NB>::PARK) {
+#pragma unroll
+                for (int ti = 0; ti < TBR; ti++)
+#pragma unroll
+                    for (int tj = 0; tj < TBC; tj++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) sh.park[(ti * TBC + tj) * 4 + r][l] = acc[ti][tj][r];
+            }
+            blk_panel<NB>(sh, pan, ni, p0, kend, tag);
+            if constexpr (BlkShared<NB>::PARK) {
+#pragma unroll
+                for (int ti = 0; ti < TBR; ti++)
+#pragma unroll
+                    for (int tj = 0; tj < TBC; tj++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) acc[ti][tj][r] = sh.park[(ti * TBC + tj) * 4 + r][l];
+            }
+        }
         __syncthreads();
         // panel tiles take the factored panel; the other tiles hand their
         // pivot-row entries to rb and clear them
 #pragma unroll
-        for (int ti = 0; ti < TB; ti++) {
+        for (int ti = 0; ti < TBR; ti++) {
             const int bi = wr + 2 * ti;
             if (bi >= NB) continue;
             int sr[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) sr[r] = sh.ipos[16 * bi + 4 * r + lg] - p0;
 #pragma unroll
-            for (int tj = 0; tj < TB; tj++) {
-                const int bj = wc + 2 * tj;
+            for (int tj = 0; tj < TBC; tj++) {
+                const int bj = wc + WC * tj;
                 if (bj >= NB) continue;
                 if (bj == pb) {
 #pragma unroll
@@ -217,15 +251,15 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
         __syncthreads();
         // rank-16 update on MFMA
 #pragma unroll
-        for (int ti = 0; ti < TB; ti++) {
+        for (int ti = 0; ti < TBR; ti++) {
             const int bi = wr + 2 * ti;
             if (bi >= NB) continue;
             double a[4];
 #pragma unroll
             for (int s4 = 0; s4 < 4; s4++) a[s4] = pan[4 * s4 + (l >> 4)][16 * bi + lc];
 #pragma unroll
-            for (int tj = 0; tj < TB; tj++) {
-                const int bj = wc + 2 * tj;
+            for (int tj = 0; tj < TBC; tj++) {
+                const int bj = wc + WC * tj;
                 if (bj >= NB || bj == pb) continue;
 #pragma unroll
                 for (int s4 = 0; s4 < 4; s4++)
@@ -236,7 +270,7 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
     }
     // inv[ipos[r]][perm[c]] = A[r][c]  ->  X[perm[c]][ipos[r]]
 #pragma unroll
-    for (int ti = 0; ti < TB; ti++) {
+    for (int ti = 0; ti < TBR; ti++) {
         const int bi = wr + 2 * ti;
         if (bi >= NB) continue;
         int qi[4];
@@ -246,8 +280,8 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
             qi[r] = i < ni ? sh.ipos[i] : -1;
         }
 #pragma unroll
-        for (int tj = 0; tj < TB; tj++) {
-            const int bj = wc + 2 * tj;
+        for (int tj = 0; tj < TBC; tj++) {
+            const int bj = wc + WC * tj;
             const int cc = 16 * bj + lc;
             if (bj >= NB || cc >= ni) continue;
             const size_t rowoff = (size_t)sh.perm[cc] * ni;
@@ -261,16 +295,17 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) /
 
 // acc = D_l - diag(cl) Xn^T diag(cn) [- diag(cl2) Xn2^T diag(cn2)] in the tile layout
 template <int NB>
-__device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 1) / 2], int ni,
+__device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC], int ni,
                                                const double *__restrict__ Dl, const double *cl, const double *Xn,
                                                const double *cn, const double *cl2, const double *Xn2,
                                                const double *cn2) {
-    constexpr int TB = (NB + 1) / 2;
+    using G = BlkGrid<NB>;
+    constexpr int TBR = G::TBR, TBC = G::TBC, WC = G::WC;
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const int wr = w >> 1, wc = w & 1;
+    const int wr = w / WC, wc = w % WC;
     const int lg = l >> 4, lc = l & 15;
 #pragma unroll
-    for (int ti = 0; ti < TB; ti++) {
+    for (int ti = 0; ti < TBR; ti++) {
         const int bi = wr + 2 * ti;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -279,8 +314,8 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[(NB + 1) / 2][(NB + 
             const double li = (Xn && rowok) ? cl[i] : 0.0;
             const double li2 = (Xn2 && rowok) ? cl2[i] : 0.0;
 #pragma unroll
-            for (int tj = 0; tj < TB; tj++) {
-                const int bj = wc + 2 * tj;
+            for (int tj = 0; tj < TBC; tj++) {
+                const int bj = wc + WC * tj;
                 const int cc = 16 * bj + lc;
                 double v = 0.0;
                 if (rowok && bj < NB && cc < ni) {

@@ -8,8 +8,9 @@
 
 namespace {
 
+// 8-wave workgroups (blocks above 112 rows) run one per CU
 template <int NB, int MINW>
-__global__ void __launch_bounds__(NT, MINW) k_factor_blk(int ni, const int *__restrict__ off, const double *__restrict__ D,
+__global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : MINW) k_factor_blk(int ni, const int *__restrict__ off, const double *__restrict__ D,
                                                     const double *__restrict__ Lsub, const double *__restrict__ Usup,
                                                     double *__restrict__ Dinv, int *__restrict__ info,
                                                     const vk_colstate *__restrict__ st) {
@@ -19,7 +20,7 @@ __global__ void __launch_bounds__(NT, MINW) k_factor_blk(int ni, const int *__re
     if (threadIdx.x == 0) sh.info = 0;
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
-    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    f64x4 acc[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC];
     // one loop for both chains (a single copy of the inversion code, which
     // is large: the kernel's instruction footprint matters with two
     // workgroups per CU): side 0 runs j = 0 .. m-1 coupled through Lsub_j /
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(NT, MINW) k_factor_blk(int ni, const int *__re
 }
 
 template <int NB, int MINW>
-__global__ void __launch_bounds__(NT, MINW) k_factor_blk_mid(int ni, const int *__restrict__ off,
+__global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : MINW) k_factor_blk_mid(int ni, const int *__restrict__ off,
                                                         const double *__restrict__ D, const double *__restrict__ Lsub,
                                                         const double *__restrict__ Usup, double *__restrict__ Dinv,
                                                         int *__restrict__ info, const vk_colstate *__restrict__ st) {
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(NT, MINW) k_factor_blk_mid(int ni, const int *
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
     const size_t l = (size_t)l0 + m;
-    f64x4 acc[(NB + 1) / 2][(NB + 1) / 2];
+    f64x4 acc[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC];
     blk_load_schur<NB>(acc, ni, D + l * nn, Lsub + l * ni, m > 0 ? Dinv + (l - 1) * nn : nullptr,
                        m > 0 ? Usup + (l - 1) * ni : nullptr, Usup + l * ni,
                        m < nz - 1 ? Dinv + (l + 1) * nn : nullptr, m < nz - 1 ? Lsub + (l + 1) * ni : nullptr);
@@ -59,9 +60,9 @@ __global__ void __launch_bounds__(NT, MINW) k_factor_blk_mid(int ni, const int *
 template <int N, int MINW>
 void launch_blk(int ni, int ncols, const int *off, const double *D, const double *Lsub, const double *Usup,
                 double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_factor_blk<N, MINW>), dim3(2 * ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+    hipLaunchKernelGGL((k_factor_blk<N, MINW>), dim3(2 * ncols), dim3(blk_nt(N)), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
                        st);
-    hipLaunchKernelGGL((k_factor_blk_mid<N, MINW>), dim3(ncols), dim3(NT), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
+    hipLaunchKernelGGL((k_factor_blk_mid<N, MINW>), dim3(ncols), dim3(blk_nt(N)), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
                        st);
 }
 
