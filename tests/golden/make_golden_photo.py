@@ -33,7 +33,10 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 NZ = 8
 
 
-def main():
+def setup_photo():
+    """Imports the reference and builds the seeded photochemical column with
+    its radiative-transfer tables; returns the objects main() and the other
+    generators work on."""
     sys.dont_write_bytecode = True
     os.environ.setdefault('MPLBACKEND', 'Agg')
     os.chdir(REF)
@@ -81,6 +84,14 @@ def main():
         rr.make_bins_read_cross(var, atm)
         make_atm.read_sflux(var, atm)
 
+    return dict(cfg=cfg, chem_funs=chem_funs, store=store, build_atm=build_atm, op=op, rng=rng, species=species,
+                ni=ni, var=var, atm=atm, make_atm=make_atm, rr=rr, solver=solver)
+
+
+def main():
+    ctx = setup_photo()
+    cfg, op, rng, species, ni = ctx['cfg'], ctx['op'], ctx['rng'], ctx['species'], ctx['ni']
+    var, atm, solver = ctx['var'], ctx['atm'], ctx['solver']
     photo_sp = sorted(var.photo_sp)
     branches = sorted(var.pho_rate_index)          # (species, branch)
     g = dict(species=np.array(species), Tco=atm.Tco, M=atm.M, dz=atm.dz.copy(),

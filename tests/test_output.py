@@ -9,6 +9,7 @@ import pytest
 
 from vulcan_amd import output as OUT
 from vulcan_amd import workloads as W
+from vulcan_amd.config import Config
 
 gpu = pytest.mark.gpu
 
@@ -94,4 +95,136 @@ def test_device_column_written_as_vul(tmp_path):
     assert data['parameter']['count'] + data['parameter']['delta_count'] + data['parameter']['nega_count'] \
         + data['parameter']['loss_count'] == int(st['attempts']) == 4
     assert data['atm']['zco'].shape == (25,) and np.all(np.diff(data['atm']['zco']) > 0)
+    b.close()
+
+
+# --------------------------------------------------------------------------
+# the structure of the reference's own .vul dictionary
+# (tests/golden/ref_vul_structure.json, make_golden_vul.py)
+# --------------------------------------------------------------------------
+
+def _describe(v):
+    if isinstance(v, np.ndarray):
+        return {'type': 'ndarray', 'shape': list(v.shape), 'dtype': str(v.dtype)}
+    if isinstance(v, dict):
+        return {'type': 'dict', 'len': len(v)}
+    if isinstance(v, (list, tuple)):
+        return {'type': 'sequence', 'len': len(v)}
+    if isinstance(v, (bool, np.bool_)):
+        return {'type': 'bool'}
+    if isinstance(v, (int, np.integer, float, np.floating)):
+        return {'type': 'number'}
+    return {'type': type(v).__name__}
+
+
+def _normalise(ref):
+    """The reference's descriptions in _describe's terms (scalars are
+    numbers, lists and tuples sequences; value types of dicts are not kept)."""
+    t = ref['type']
+    if t == 'ndarray':
+        return {'type': t, 'shape': ref['shape'], 'dtype': ref['dtype']}
+    if t == 'dict':
+        return {'type': t, 'len': ref['len']}
+    if t in ('list', 'tuple'):
+        return {'type': 'sequence', 'len': ref['len']}
+    if t in ('float', 'float64', 'int', 'int64'):
+        return {'type': 'number'}     # (e.g. var.t is the integer 0 before the first step)
+    return {'type': t}
+
+
+def _structure():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, 'golden', 'ref_vul_structure.json')) as f:
+        return json.load(f)
+
+
+def _compare(ours, ref, skip_len=(), lead_free=()):
+    """Key sets equal; every entry of the same type, arrays of the same shape
+    and dtype (the leading dimension of `lead_free` entries may differ: the
+    number of saved steps), containers of the same length (except
+    `skip_len`)."""
+    assert set(ours) == set(ref), (sorted(set(ours) ^ set(ref)))
+    for k in ref:
+        a, b = _describe(ours[k]), _normalise(ref[k])
+        if k in skip_len:
+            a.pop('len', None), b.pop('len', None)
+        if k in lead_free and a.get('shape'):
+            a['shape'], b['shape'] = a['shape'][1:], b['shape'][1:]
+        assert a == b, (k, a, b)
+
+
+@gpu
+def test_device_vul_has_the_reference_structure():
+    """A device run of the golden column written as the reference's
+    save_out writes it: the same keys in 'variable', 'atm' and 'parameter',
+    the same array shapes and dtypes and container lengths.  (The
+    reference's run kept an empty Rf: its generated chem_funs set-up does not
+    run ReadRate.)"""
+    import torch
+    from _golden_util import load, network_from, column_from, traj_config
+    from vulcan_amd.device import ColumnBatch, DevicePlan
+    assert torch.cuda.is_available()
+    ref = _structure()
+    meta = ref['thermo_meta']
+    d = load()
+    net, col = network_from(d), column_from(d)
+    assert (col.nz, net.ni, net.nr) == (meta['nz'], meta['ni'], meta['nr'])
+    cfg = traj_config(d)
+    b = ColumnBatch(DevicePlan(net, alpha=d['alpha']), [col], [d['y0']], cfg, history=cfg.conv_step + 1,
+                    k=d['k'], tune=False)
+    for _ in range(20):
+        b.integrate(8)
+        if int(b.host_state()['done'][0]):
+            break
+    cnt = int(b.host_state()['count'][0])
+    t_time, y_time = b.read_history(max(0, cnt - cfg.conv_step - 1), cnt)
+    var, atm, para = OUT.column_result(b, 0, col, d['y0'], y_time=list(y_time), t_time=list(t_time[0]))
+    data = OUT.save_dict(var, atm, para, net.species, net.nr, save_evolution=True,
+                         save_evo_frq=int(meta['save_evo_frq']))
+    _compare(data['variable'], ref['thermo']['variable'], skip_len=('Rf',), lead_free=('y_time', 't_time'))
+    _compare(data['atm'], ref['thermo']['atm'])
+    _compare(data['parameter'], ref['thermo']['parameter'])
+    b.close()
+
+
+@gpu
+def test_device_vul_photochemistry_keys_have_the_reference_structure():
+    """With the radiative transfer, var carries the reference's photochemistry
+    keys (store.py:88-89) with its shapes: tau / sflux on nz + 1 levels,
+    aflux per layer, the cross-section dictionaries per species and branch,
+    J_sp with the per-species totals (sp, 0), n_branch."""
+    import torch
+    from _golden_util import load, network_from, column_from
+    from test_photo import _tables_from_golden
+    from vulcan_amd import photo as P
+    from vulcan_amd.device import ColumnBatch, DevicePlan
+    assert torch.cuda.is_available()
+    ref = _structure()
+    meta = ref['photo_meta']
+    pg = load('ref_photo_CHO_nz8.npz')
+    d = load()
+    net, col = network_from(d), column_from(d)
+    t = _tables_from_golden(pg, list(net.species))
+    br = [(str(s), int(n)) for s, n in zip(pg['branch_sp'], pg['branch_no'])]
+    t.branches = br
+    t.n_branch = {sp: max(n for s, n in br if s == sp) for sp in {s for s, _ in br}}
+    b = ColumnBatch(DevicePlan(net, alpha=d['alpha']), [col], [pg['c0_y']], Config(use_photo=True),
+                    k=d['k'], tune=False)
+    ph = P.DevicePhoto(b.plan, b.layout, t)
+    b.photo = ph
+    b.update_photo(force=True)
+    var, atm, para = OUT.column_result(b, 0, col, pg['c0_y'], photo=ph)
+    data = OUT.save_dict(var, atm, para, net.species, net.nr, use_photo=True)
+    assert (col.nz, len(t.bins)) == (meta['nz'], meta['nbin'])
+    # (the photochemical set-up read its rates from the C-H-O photochemical
+    # network file and monitored one more element than the golden network:
+    # the lengths of k and of the atom dictionaries are configuration)
+    _compare(data['variable'], ref['photo']['variable'],
+             skip_len=('Rf', 'k', 'atom_ini', 'atom_sum', 'atom_loss', 'atom_conden'))
+    # J_sp's totals are the sums of the branches
+    for sp, n in t.n_branch.items():
+        assert np.allclose(var.J_sp[(sp, 0)], sum(var.J_sp[(sp, q)] for q in range(1, n + 1)), rtol=1e-14, atol=0)
+    ph.close()
     b.close()

@@ -140,16 +140,23 @@ class Ros2:
 
     def compute_J(self, var, atm):
         """Photolysis rates of the last device update into var.k (op.py:2751-2787):
-        k = J f_diurnal at the reference's pho_rate_index, var.J_sp[(sp, br)] = J."""
+        k = J f_diurnal at the reference's pho_rate_index; var.J_sp[(sp, br)] = J
+        for every branch br >= 1 and var.J_sp[(sp, 0)] their sum, in branch
+        order.  Branches left out of the device tables (remove_list) keep
+        J = 0 in J_sp."""
         t = self._photo_dev.tables
-        cols = set(t.br_col)
+        col = {b: c for b, c in zip(t.branches, t.br_col)}
         for sp_br, idx in var.pho_rate_index.items():
-            if idx - 1 in cols:
-                var.k[idx] = self._photo_k[:, idx - 1].copy()
-        var.J_sp = {}
-        for sp_br, idx in var.pho_rate_index.items():
-            if idx - 1 in cols:
-                var.J_sp[sp_br] = self._photo_k[:, idx - 1] / self.cfg.f_diurnal
+            if sp_br in col:
+                var.k[idx] = self._photo_k[:, col[sp_br]].copy()
+        nz = np.shape(var.y)[0]
+        n_branch = getattr(var, 'n_branch', None) or t.n_branch
+        var.J_sp = {(sp, bn): np.zeros(nz) for sp in var.photo_sp for bn in range(n_branch[sp] + 1)}
+        for sp in var.photo_sp:
+            for bn in range(1, n_branch[sp] + 1):
+                if (sp, bn) in col:
+                    var.J_sp[(sp, bn)] = self._photo_k[:, col[(sp, bn)]] / self.cfg.f_diurnal
+                var.J_sp[(sp, 0)] += var.J_sp[(sp, bn)]
 
     def compute_Jion(self, var, atm):
         raise NotImplementedError('photoionisation (use_ion) is outside the accelerated path')

@@ -5,9 +5,11 @@ A `.vul` file is the pickle (protocol 4) of
     {'variable': var_save, 'atm': vars(atm), 'parameter': vars(para)}
 
 where var_save holds 'species', 'nr', the keys of Variables.var_save
-(store.py:86-88, without the photochemistry ones: radiative transfer is not
-run here) and, with save_evolution, every save_evo_frq-th entry of the keys
-of Variables.var_evol_save (store.py:92; op.py:3241-3247).  VULCAN's
+(store.py:86-88, with the photochemistry extension when the radiative
+transfer ran, store.py:88-89) and, with save_evolution, every
+save_evo_frq-th entry of the keys of Variables.var_evol_save (store.py:92;
+op.py:3241-3247).  'atm' and 'parameter' carry every attribute of VULCAN's
+AtmData and Parameters (store.py:113-208) with their shapes.  VULCAN's
 plot_py readers index it as data['variable']['ymix'][:, species.index(sp)],
 data['atm']['pco'], data['atm']['zco'], data['atm']['zmco'],
 data['atm']['n_0'], data['variable']['t_time'] and
@@ -27,18 +29,44 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-# Variables.var_save without the photochemistry extension (store.py:86-90)
+# Variables.var_save (store.py:86-87)
 VAR_SAVE = ['k', 'y', 'ymix', 'y_ini', 't', 'dt', 'longdy', 'longdydt', 'atom_ini', 'atom_sum', 'atom_loss',
             'atom_conden', 'aflux_change', 'Rf']
+# its photochemistry extension (store.py:88-89), and with T_cross_sp
+VAR_SAVE_PHOTO = ['nbin', 'bins', 'dbin1', 'dbin2', 'tau', 'sflux', 'aflux', 'cross', 'cross_scat', 'cross_J', 'J_sp',
+                  'n_branch']
+VAR_SAVE_T_CROSS = ['cross_J', 'cross_T']
 # Variables.var_evol_save (store.py:92)
 VAR_EVOL_SAVE = ['y_time', 't_time']
 
+# Parameters.tableau20 (store.py:198-206): RGB scaled to [0, 1]
+_TABLEAU20 = [(31, 119, 180), (255, 127, 14), (44, 160, 44), (214, 39, 40), (148, 103, 189), (140, 86, 75),
+              (227, 119, 194), (127, 127, 127), (188, 189, 34), (23, 190, 207), (174, 199, 232), (255, 187, 120),
+              (152, 223, 138), (255, 152, 150), (197, 176, 213), (196, 156, 148), (247, 182, 210), (199, 199, 199),
+              (219, 219, 141), (158, 218, 229)]
+TABLEAU20 = [(r / 255., g / 255., b / 255.) for r, g, b in _TABLEAU20]
+
+
+def var_save_keys(use_photo: bool = False, T_cross: bool = False):
+    keys = list(VAR_SAVE)
+    if use_photo:
+        keys += VAR_SAVE_PHOTO
+        if T_cross:
+            keys += VAR_SAVE_T_CROSS
+    return keys
+
 
 def save_dict(var, atm, para, species: Sequence[str], nr: int, save_evolution: bool = False,
-              save_evo_frq: int = 10) -> dict:
-    """The dictionary Output.save_out pickles (op.py:3237-3254)."""
+              save_evo_frq: int = 10, use_photo: Optional[bool] = None, T_cross: Optional[bool] = None) -> dict:
+    """The dictionary Output.save_out pickles (op.py:3237-3254).  The
+    photochemistry keys are saved when `use_photo` (default: when `var`
+    holds the radiative-transfer results)."""
+    if use_photo is None:
+        use_photo = hasattr(var, 'tau')
+    if T_cross is None:
+        T_cross = bool(getattr(var, 'cross_T', None))
     var_save = {'species': list(species), 'nr': int(nr)}
-    for key in VAR_SAVE:
+    for key in var_save_keys(use_photo, T_cross):
         var_save[key] = getattr(var, key)
     if save_evolution:
         for key in VAR_EVOL_SAVE:
@@ -47,10 +75,10 @@ def save_dict(var, atm, para, species: Sequence[str], nr: int, save_evolution: b
 
 
 def save_vul(path: str, var, atm, para, species: Sequence[str], nr: int, save_evolution: bool = False,
-             save_evo_frq: int = 10, human_readable: bool = False) -> str:
+             save_evo_frq: int = 10, human_readable: bool = False, use_photo: Optional[bool] = None) -> str:
     """Write `path` as VULCAN does (pickle protocol 4, or the str() of the
     dictionary for output_humanread, op.py:3249-3254).  Returns the path."""
-    d = save_dict(var, atm, para, species, nr, save_evolution, save_evo_frq)
+    d = save_dict(var, atm, para, species, nr, save_evolution, save_evo_frq, use_photo)
     out_dir = os.path.dirname(os.path.abspath(path))
     os.makedirs(out_dir, exist_ok=True)
     if human_readable:
@@ -66,15 +94,49 @@ def _atom_dict(atoms, values) -> dict:
     return {a: float(v) for a, v in zip(atoms, values)}
 
 
+def _photo_var(batch, c: int, photo, nz: int) -> dict:
+    """The photochemistry entries of var (store.py:88-89) for column c from
+    a DevicePhoto's tables and last update."""
+    from . import photo as P
+    t = photo.tables
+    net = batch.plan.net
+    sp = net.species
+    o0 = int(batch.layout.off[c])
+    lev = slice(o0 + c, o0 + c + nz + 1)             # nz + 1 levels per column
+    lay = slice(o0, o0 + nz)
+    k = batch.column('k', c)
+    f_d = t.f_diurnal
+    n_branch = dict(t.n_branch or {})
+    branches = list(t.branches or [])
+    col = dict(zip(branches, t.br_col))
+    J = {(s, b): np.zeros(nz) for s in n_branch for b in range(n_branch[s] + 1)}
+    for s in n_branch:
+        for b in range(1, n_branch[s] + 1):
+            if (s, b) in col:
+                J[(s, b)] = k[:, col[(s, b)]] / f_d
+            J[(s, 0)] = J[(s, 0)] + J[(s, b)]
+    out = dict(nbin=len(t.bins), bins=np.array(t.bins), dbin1=t.dbin1, dbin2=t.dbin2,
+               tau=photo.read(P.VK_PHOTO_TAU)[lev], sflux=photo.read(P.VK_PHOTO_SFLUX)[lev],
+               aflux=photo.read(P.VK_PHOTO_AFLUX)[lay],
+               cross={sp[i]: t.cross[a] for a, i in enumerate(t.abs_sp)},
+               cross_scat={sp[i]: t.cross_scat[a] for a, i in enumerate(t.scat_sp)},
+               cross_J={b: t.cross_J[q] for q, b in enumerate(branches)}, J_sp=J, n_branch=n_branch)
+    T_sp = [sp[i] for a, i in enumerate(t.abs_sp) if t.abs_T[a] >= 0]
+    if T_sp:
+        out['cross_T'] = {s: t.cross_T[t.abs_T[t.abs_sp.index(sp.index(s))]][lay] for s in T_sp}
+    return out
+
+
 def column_result(batch, c: int, col, y_ini: np.ndarray, y_time: Optional[np.ndarray] = None,
-                  t_time: Optional[np.ndarray] = None):
+                  t_time: Optional[np.ndarray] = None, photo=None):
     """(var, atm, para) of column `c` of a ColumnBatch, in VULCAN's layout.
 
     `col` is the column's atmosphere (vulcan_amd.atmosphere.Column) and
     `y_ini` its initial number densities (nz, ni).  The arrays that the
     integration updates on the device (update_mu_dz: g, Hp, dz, mu, zco,
     dzi, Hpi) are read back from the batch; y_time / t_time, when given,
-    become the evolution arrays.
+    become the evolution arrays; with `photo` (the batch's DevicePhoto) var
+    also holds the radiative-transfer results.
     """
     net = batch.plan.net
     ni, nz = net.ni, col.nz
@@ -97,6 +159,10 @@ def column_result(batch, c: int, col, y_ini: np.ndarray, y_time: Optional[np.nda
         aflux_change=0.0,
         Rf={2 * p + 1: net.equation(p) for p in range(net.npairs)},
         y_time=[] if y_time is None else y_time, t_time=[] if t_time is None else t_time)
+    if photo is not None:
+        for key, v in _photo_var(batch, c, photo, nz).items():
+            setattr(var, key, v)
+        var.aflux_change = float(st['aflux_change'])
     zco = batch.zco.cpu().numpy()
     zo = o0 + c                      # zco holds nz + 1 entries per column
     zc = zco[zo:zo + nz + 1]
@@ -107,12 +173,25 @@ def column_result(batch, c: int, col, y_ini: np.ndarray, y_time: Optional[np.nda
         Kzz=np.array(col.Kzz), vz=np.array(col.vz), M=M, n_0=M.copy(), Hp=batch.column('Hp', c),
         mu=batch.column('mu', c), ms=np.array(col.ms), Dzz=np.array(col.Dzz), alpha=np.array(col.alpha),
         gs=float(col.gs), g=batch.column('g', c), Ti=np.array(col.Ti), Hpi=batch.column('Hpi', c)[:nz - 1],
-        top_flux=np.zeros(ni), bot_flux=np.zeros(ni), bot_vdep=np.zeros(ni), bot_fix_sp=np.zeros(ni),
+        top_flux=_per_column(batch, 'top_flux', c, ni, col.top_flux),
+        bot_flux=_per_column(batch, 'bot_flux', c, ni, col.bot_flux),
+        bot_vdep=_per_column(batch, 'bot_vdep', c, ni, col.bot_vdep), bot_fix_sp=np.zeros(ni),
+        Dzz_cen=np.zeros((nz, ni)), vm=np.zeros((nz, ni)) if col.vm is None else np.array(col.vm),
+        vs=np.zeros((nz - 1, ni)), pref_indx=int(col.pref_indx), gas_indx=list(range(ni)), fix_sp_indx={},
         sat_p={}, sat_mix={}, conden_min_lev={}, r_p={}, rho_p={})
     para = SimpleNamespace(
         nega_y=float(st['nega_y']), small_y=float(st['small_y']), delta=float(st['delta']),
         count=int(st['count']), nega_count=int(st['nega_count']), loss_count=int(st['loss_count']),
         delta_count=int(st['delta_count']), end_case=int(st['end_case']), solver_str='solver',
         switch_final_photo_frq=False, where_varies_most=np.zeros((nz, ni)), pic_count=0,
-        fix_species_start=False)
+        fix_species_start=False, start_time=0.0, tableau20=list(TABLEAU20))
     return var, atm, para
+
+
+def _per_column(batch, name, c, ni, host):
+    """A [ncols][ni] boundary array of the batch (its device copy when it
+    has one), or the column's own value, or zeros."""
+    t = getattr(batch, name, None)
+    if t is not None:
+        return t.cpu().numpy()[c].copy()
+    return np.zeros(ni) if host is None else np.array(host, dtype=np.float64)

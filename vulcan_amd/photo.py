@@ -82,6 +82,10 @@ class PhotoTables:
     f_diurnal: float
     flux_atol: float
     hc: float = HC
+    # (species, branch) of each device branch, and the branch count of every
+    # photolysed species (VULCAN's var.n_branch): J_sp and the .vul keys
+    branches: Optional[List[Tuple[str, int]]] = None
+    n_branch: Optional[Dict[str, int]] = None
 
 
 def photo_branches(network_file: str) -> Dict[Tuple[str, int], int]:
@@ -225,7 +229,8 @@ def build_tables(net: nw.Network, network_file: str, Tco_layers: np.ndarray, pcf
                        cross_T=cross_T, scat_sp=scat, cross_scat=cross_scat, br_col=br_col,
                        br_T=[TJ.index(q) if q in TJ else -1 for q in keep], cross_J=cross_J[keep],
                        cross_J_T=cross_J_T, dbin1=pcfg.dbin1, dbin2=pcfg.dbin2, cos_sl=float(np.cos(pcfg.sl_angle)),
-                       edd=pcfg.edd, f_diurnal=pcfg.f_diurnal, flux_atol=pcfg.flux_atol)
+                       edd=pcfg.edd, f_diurnal=pcfg.f_diurnal, flux_atol=pcfg.flux_atol,
+                       branches=[blist[q] for q in keep], n_branch=dict(n_branch))
 
 
 def _log10_or(v):
@@ -297,7 +302,10 @@ def tables_from_var(var, species: Sequence[str], nr: int, cfg) -> PhotoTables:
                        cross_J=np.stack([var.cross_J[b] for b in blist]),
                        cross_J_T=stackT([var.cross_J_T[blist[q]] for q in TJ]),
                        dbin1=float(var.dbin1), dbin2=float(var.dbin2), cos_sl=float(np.cos(cfg.sl_angle)),
-                       edd=float(cfg.edd), f_diurnal=float(cfg.f_diurnal), flux_atol=float(cfg.flux_atol))
+                       edd=float(cfg.edd), f_diurnal=float(cfg.f_diurnal), flux_atol=float(cfg.flux_atol),
+                       branches=list(blist),
+                       n_branch=dict(getattr(var, 'n_branch', None) or
+                                     {sp: max(b for s, b in var.pho_rate_index if s == sp) for sp in photo_sp}))
 
 
 class DevicePhoto:
