@@ -1,6 +1,6 @@
 """Ros2 column-steps/s on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-mixed] [--no-steady]
 
 A step is one accepted Ros2 step of the column (one iteration of VULCAN's
 integration loop, op.py:813-923), rejected attempts included in the time.
@@ -26,7 +26,14 @@ The JSON line also carries
 * ``batched``: the same network on 256 columns with nz uniform on [80, 200]
   (one captured launch sequence per attempt for all columns);
 * ``mixed``: 512 columns over three networks (ni = 40, 93, 150), one batch
-  per network on its own stream, block-inversion kernel tuned per ni.
+  per network on its own stream, block-inversion kernel tuned per ni;
+* ``steady_state``: the other half of the metric, time to steady state from
+  a warm and from a cold start: held-out seeded columns of the same network
+  (nz = 150, random T profiles, Kzz and element abundances) run to VULCAN's
+  own ``conv()`` criteria (defaults) in one batch each, once from chemical
+  equilibrium and once from the committed warm-start network's prediction
+  (WARMSTART_MODEL below, projected on each column's element inventory);
+  medians over the columns that converge in both runs within the attempt cap.
 """
 import argparse
 import json
@@ -49,6 +56,10 @@ FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
 # per run, each scaled by its reading for a 512 MiB device copy); the file
 # gives the command that made it.
 TRAFFIC_FILE = os.path.join('profiles', 'hbm_traffic_ros2_attempt.md')
+# the warm-start network (307 -> 4 x 1024 SiLU -> 93 x 150), trained on the
+# device on steady states the batched solver generated (python -m
+# vulcan_amd.warmstart_study generate / train); plain arrays, BF16 weights
+WARMSTART_MODEL = os.path.join('vulcan_amd', 'weights', 'warmstart_ni93_nz150.npz')
 
 
 def factor_traffic(section):
@@ -211,6 +222,19 @@ def bench_mixed(attempts, warmup, ncols=512):
     return dict(time=dt, col_steps=int(c1 - c0), attempts=attempts, ncols=ncols, groups=groups)
 
 
+def bench_steady(columns, max_attempts, poll=32):
+    """Median time to steady state, cold vs warm, on `columns` held-out columns."""
+    from vulcan_amd.warmstart_study import evaluate
+    path = os.path.join(ROOT, WARMSTART_MODEL)
+    if not os.path.exists(path):
+        return {'skipped': 'no trained network at %s' % WARMSTART_MODEL}
+    t0 = time.perf_counter()
+    rep = evaluate(path, columns, 6006, max_attempts, poll)
+    rep['leg_seconds'] = time.perf_counter() - t0
+    rep['model'] = WARMSTART_MODEL
+    return rep
+
+
 CPU_SNIPPET = r'''
 import json, sys, time
 sys.path.insert(0, %(root)r)
@@ -348,6 +372,9 @@ def main():
     ap.add_argument('--cpu-budget', type=float, default=20.0)
     ap.add_argument('--no-cpu-aggregate', action='store_true')
     ap.add_argument('--no-graph', action='store_true', help='eager launches instead of the captured HIP graph')
+    ap.add_argument('--steady-columns', type=int, default=128)
+    ap.add_argument('--steady-attempts', type=int, default=4000, help='attempt cap of the steady-state runs')
+    ap.add_argument('--no-steady', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has one MI355X and the columns are '
@@ -411,6 +438,17 @@ def main():
         out['mixed'] = {'workload': '512 columns over ni=40/93/150 networks (200/600/1100 pairs), nz uniform [100,150]',
                         'value': mx['col_steps'] / mx['time'], 'unit': 'column-steps/s', 'attempts': mx['attempts'],
                         'col_steps': mx['col_steps'], 'time_s': mx['time'], 'groups': mx['groups']}
+    if not args.no_steady:
+        print('[bench] time to steady state, cold vs warm: %d columns, cap %d attempts'
+              % (args.steady_columns, args.steady_attempts), file=sys.stderr, flush=True)
+        st = bench_steady(args.steady_columns, args.steady_attempts)
+        st['workload'] = ('%d held-out columns (seed 6006) of the ni=93 network, nz=150, random T profile, Kzz and '
+                          'element abundances; cold = chemical equilibrium, warm = warm-start network prediction '
+                          'projected on the element inventory; VULCAN default conv() criteria; one batch per run'
+                          % args.steady_columns)
+        if st.get('speedup_median_wall'):
+            st['metric'] = 'median time-to-steady-state, cold / warm (wall s)'
+        out['steady_state'] = st
     if not args.no_cpu:
         print('[bench] cpu baseline (%.0f s budget)' % args.cpu_budget, file=sys.stderr, flush=True)
         out['cpu_baseline'] = cpu_baseline(args.cpu_budget)

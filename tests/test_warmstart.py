@@ -11,6 +11,8 @@ criteria (count_min lowered to 20 so that the step count reflects the
 approach to steady state rather than the count floor).  The full-size
 study (93 species, 150 layers) is vulcan_amd/warmstart_study.py.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -21,6 +23,8 @@ from vulcan_amd.workloads import SOLAR, iso_gradient_profile
 
 gpu = pytest.mark.gpu
 NZ = 20
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     'vulcan_amd', 'weights', 'warmstart_ni93_nz150.npz')
 
 
 def _torch():
@@ -88,3 +92,84 @@ def test_trained_warm_start_reaches_steady_state_sooner():
     dev = [np.max(np.abs(np.log10(np.maximum(cold.ymix[c], 1e-30)) - np.log10(np.maximum(warm.ymix[c], 1e-30)))
                   [cold.ymix[c] > 1e-10]) for c in np.nonzero(both)[0]]
     assert np.median(dev) < 0.1
+
+
+def test_bf16_bits_round_to_nearest_even():
+    import torch
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 0.0, -0.0], np.float32)
+    assert [int(b) for b in WS.bf16_bits(x)] == [0x3F80, 0x3F80, 0x3F82, 0xC020, 0x0000, 0x8000]
+    rng = np.random.default_rng(3)
+    w = (rng.standard_normal(20000) * 10.0 ** rng.uniform(-30, 30, 20000)).astype(np.float32)
+    ref = torch.from_numpy(w).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(WS.bf16_bits(w), ref)
+    back = torch.from_numpy(ref.view(np.int16)).view(torch.bfloat16).float().numpy()
+    assert np.array_equal(WS.bf16_to_f32(ref), back)
+
+
+class _Net:
+    """What save_model reads of a network: its widths and host arrays."""
+    def __init__(self, sizes, seed):
+        rng = np.random.default_rng(seed)
+        self.sizes = list(sizes)
+        self.W = [rng.standard_normal((a, b)).astype(np.float32) for a, b in zip(sizes[:-1], sizes[1:])]
+        self.b = [rng.standard_normal(b).astype(np.float32) for b in sizes[1:]]
+
+    def state_dict(self):
+        return {'sizes': self.sizes, 'W': self.W, 'b': self.b}
+
+
+def test_network_file_round_trip(tmp_path):
+    net = _Net([7, 16, 16, 5], 4)
+    rng = np.random.default_rng(5)
+    norm = (rng.standard_normal(7), rng.uniform(1, 2, 7), rng.standard_normal(5), rng.uniform(1, 2, 5))
+    path = str(tmp_path / 'net.npz')
+    WS.save_model(path, net, norm)
+    d = WS.read_model(path)
+    assert d['sizes'] == net.sizes
+    for i in range(3):
+        assert np.array_equal(d['W%d' % i], WS.bf16_bits(net.W[i]))
+        assert np.array_equal(d['b%d' % i], net.b[i])
+        assert np.max(np.abs(WS.bf16_to_f32(d['W%d' % i]) - net.W[i]) / np.abs(net.W[i])) <= 2.0 ** -8
+    for k, v in zip(('xm', 'xs', 'ym', 'ys'), norm):
+        assert np.array_equal(d[k], v)
+    # a layer whose shape disagrees with the widths is refused
+    net.W[1] = net.W[1][:, :8]
+    WS.save_model(path, net, norm)
+    with pytest.raises(ValueError):
+        WS.read_model(path)
+
+
+def test_committed_network_file():
+    if not os.path.exists(MODEL):
+        pytest.skip('no committed warm-start network')
+    d = WS.read_model(MODEL)
+    assert d['sizes'] == [2 * WS.NZ + 7] + [1024] * 4 + [93 * WS.NZ]
+    for k in ('xm', 'xs', 'ym', 'ys'):
+        assert np.all(np.isfinite(d[k]))
+    assert np.all(d['xs'] > 0) and np.all(d['ys'] > 0)
+    for i in range(5):
+        assert np.all(np.isfinite(WS.bf16_to_f32(d['W%d' % i]))) and np.all(np.isfinite(d['b%d' % i]))
+
+
+@gpu
+def test_committed_network_loads_and_seeds_valid_starts():
+    """The device network holds exactly the stored BF16 weights, and its
+    prediction for held-out columns, projected, is a composition with each
+    column's element inventory."""
+    _torch()
+    if not os.path.exists(MODEL):
+        pytest.skip('no committed warm-start network')
+    from vulcan_amd.warmstart_study import predict
+    from vulcan_amd.workloads import full_network
+    mlp, norm = WS.load_model(MODEL)
+    d = WS.read_model(MODEL)
+    for i, wb in enumerate(mlp.Wb):
+        assert np.array_equal(wb.cpu().numpy().view(np.uint16), d['W%d' % i])
+    net = full_network()
+    specs = WS.sample_specs(3, 6006, WS.NZ)
+    P = predict(mlp, norm, WS.features(specs))
+    assert P.shape == (3, WS.NZ * net.ni) and np.all(np.isfinite(P))
+    for s, p in zip(specs, P):
+        c = WS.warm_case((net, s, p))
+        y = c.y0 / c.col.M[:, None]
+        assert np.all(y >= 0) and np.allclose(y.sum(axis=1), 1.0, rtol=1e-10)

@@ -161,6 +161,59 @@ def targets(ymix: np.ndarray) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
+# a trained network on disk: plain arrays in an .npz (no pickled objects)
+# --------------------------------------------------------------------------
+
+def bf16_bits(w: np.ndarray) -> np.ndarray:
+    """float32 -> BF16 bit patterns (uint16), rounding to nearest even as the
+    device conversion does (finite inputs)."""
+    u = np.ascontiguousarray(w, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def bf16_to_f32(bits: np.ndarray) -> np.ndarray:
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+def save_model(path: str, mlp, norm) -> None:
+    """Layer widths, weights as the BF16 values the GEMMs read, FP32 biases
+    and the FP64 feature and target normalisation (mean, std)."""
+    sd = mlp.state_dict()
+    arrs = {'sizes': np.asarray(sd['sizes'], np.int64)}
+    for i, (w, b) in enumerate(zip(sd['W'], sd['b'])):
+        arrs['W%d' % i] = bf16_bits(w)
+        arrs['b%d' % i] = np.asarray(b, np.float32)
+    for k, v in zip(('xm', 'xs', 'ym', 'ys'), norm):
+        arrs[k] = np.asarray(v, np.float64)
+    np.savez_compressed(path, **arrs)
+
+
+def read_model(path: str) -> dict:
+    """The arrays of a saved network, checked against its layer widths."""
+    with np.load(path, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    sizes = [int(s) for s in d['sizes']]
+    for i, (fi, fo) in enumerate(zip(sizes[:-1], sizes[1:])):
+        if d['W%d' % i].shape != (fi, fo) or d['b%d' % i].shape != (fo,):
+            raise ValueError('%s: layer %d does not have the shape its widths give' % (path, i))
+    if d['xm'].shape != (sizes[0],) or d['ym'].shape != (sizes[-1],):
+        raise ValueError('%s: normalisation does not match the layer widths' % path)
+    d['sizes'] = sizes
+    return d
+
+
+def load_model(path: str):
+    """(MLP on the device, normalisation) of a saved network."""
+    from .mlp import MLP
+    d = read_model(path)
+    n = len(d['sizes']) - 1
+    mlp = MLP(d['sizes'])
+    mlp.load_state_dict({'W': [bf16_to_f32(d['W%d' % i]) for i in range(n)],
+                         'b': [d['b%d' % i] for i in range(n)]})
+    return mlp, (d['xm'], d['xs'], d['ym'], d['ys'])
+
+
+# --------------------------------------------------------------------------
 # batched runs to steady state with per-column completion times
 # --------------------------------------------------------------------------
 

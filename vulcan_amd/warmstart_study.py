@@ -4,6 +4,8 @@ time-to-steady-state of warm and cold starts on held-out columns.
     python -m vulcan_amd.warmstart_study generate --first 0 --count 1024 --out gen0.npz
     python -m vulcan_amd.warmstart_study generate --first 1024 --count 2048 --data gen0.npz --out gen1.npz
     python -m vulcan_amd.warmstart_study train-eval --data gen0.npz gen1.npz --test 512 --out report.json
+    python -m vulcan_amd.warmstart_study train --data gen0.npz gen1.npz --out model.npz
+    python -m vulcan_amd.warmstart_study evaluate --model model.npz --test 512 --out report.json
 
 1. ``generate``: seeded columns (sampler in vulcan_amd/warmstart.py; column
    i of the training stream is the same whatever the chunking) run to
@@ -22,11 +24,16 @@ time-to-steady-state of warm and cold starts on held-out columns.
    projected MLP prediction.  Reported: median wall time to steady state
    per column (completion observed at the host poll), accepted steps and
    attempts, convergence counts and the agreement of the two steady states.
+3. ``train`` trains as ``train-eval`` does and saves the network
+   (``warmstart.save_model``: BF16 weights, FP32 biases, the feature and
+   target normalisation); ``evaluate`` runs ``train-eval``'s comparison
+   with a saved network.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -38,7 +45,7 @@ from .workloads import _pool_map, full_network
 
 
 def log(msg):
-    print('[warmstart] ' + msg, flush=True)
+    print('[warmstart] ' + msg, file=sys.stderr, flush=True)
 
 
 def _cases(fn, net, items):
@@ -104,6 +111,7 @@ def generate(first, count, seed, nz, batch, max_attempts, poll, out, data=None, 
             first + b0, first + n_done, int(res.done.sum()), len(chunk),
             int(np.median(res.attempts[res.done])) if res.done.any() else None, time.perf_counter() - t0))
     sl = slice(0, n_done)
+    Y[~rec['done']] = 0            # only converged columns are training pairs; zero rows compress away
     np.savez_compressed(out, index=np.arange(first, first + n_done, dtype=np.int32), X=X[sl], Y=Y[sl], seed=seed,
                         nz=nz, **{k: v[sl] for k, v in rec.items()})
     log('generated %d columns (%d converged) in %.1f s -> %s' % (n_done, int(rec['done'][sl].sum()),
@@ -192,10 +200,7 @@ def train_eval(data, test, test_seed, hidden, layers, epochs, mb, lr, max_attemp
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     if save_model:
-        sd = mlp.state_dict()
-        np.savez_compressed(save_model, **{'W%d' % i: w for i, w in enumerate(sd['W'])},
-                            **{'b%d' % i: b for i, b in enumerate(sd['b'])},
-                            xm=norm[0], xs=norm[1], ym=norm[2], ys=norm[3], sizes=np.array(sizes))
+        WS.save_model(save_model, mlp, norm)
     specs = WS.sample_specs(test, test_seed, nz)
     Xt = WS.features(specs)
     t2 = time.perf_counter()
@@ -211,6 +216,38 @@ def train_eval(data, test, test_seed, hidden, layers, epochs, mb, lr, max_attemp
     log(json.dumps(rep))
     with open(out, 'w') as f:
         json.dump(rep, f, indent=1)
+    return rep
+
+
+def train_only(data, hidden, layers, epochs, mb, lr, out):
+    """Train on the converged columns of `data` and save the network."""
+    X, Y, info = load_data(data)
+    log('training on %d converged columns of %d generated' % (X.shape[0], info['generated']))
+    sizes = [X.shape[1]] + [hidden] * layers + [Y.shape[1]]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mlp, norm, final_loss = train(sizes, X, Y, epochs, mb, lr)
+    torch.cuda.synchronize()
+    WS.save_model(out, mlp, norm)
+    log('trained %s in %.1f s, final normalised loss %.4e -> %s' % (sizes, time.perf_counter() - t1, final_loss,
+                                                                   out))
+    return final_loss
+
+
+def evaluate(model, test, test_seed, max_attempts, poll, nz=WS.NZ):
+    """Cold and warm runs to steady state of `test` held-out columns with a
+    saved network (the comparison of train-eval without the training)."""
+    net = full_network()
+    mlp, norm = WS.load_model(model)
+    specs = WS.sample_specs(test, test_seed, nz)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P = predict(mlp, norm, WS.features(specs))
+    t_pred = time.perf_counter() - t0
+    from .device import DevicePlan
+    rep = compare(net, specs, P, max_attempts, poll, plan=DevicePlan(net))
+    rep.update({'predict_seconds': t_pred, 'mlp_sizes': list(mlp.sizes), 'nz': nz, 'ni': net.ni,
+                'test_seed': test_seed, 'max_attempts': max_attempts})
     return rep
 
 
@@ -243,10 +280,32 @@ def main(argv=None):
     t.add_argument('--poll', type=int, default=16)
     t.add_argument('--out', required=True)
     t.add_argument('--save-model', default=None)
+    o = sub.add_parser('train')
+    o.add_argument('--data', nargs='+', required=True)
+    o.add_argument('--hidden', type=int, default=1024)
+    o.add_argument('--layers', type=int, default=4)
+    o.add_argument('--epochs', type=int, default=300)
+    o.add_argument('--mb', type=int, default=256)
+    o.add_argument('--lr', type=float, default=1e-3)
+    o.add_argument('--out', required=True, help='.npz of the trained network')
+    e = sub.add_parser('evaluate')
+    e.add_argument('--model', required=True)
+    e.add_argument('--test', type=int, default=512)
+    e.add_argument('--test-seed', type=int, default=6006)
+    e.add_argument('--max-attempts', type=int, default=8000)
+    e.add_argument('--poll', type=int, default=16)
+    e.add_argument('--out', required=True)
     a = ap.parse_args(argv)
     if a.cmd == 'generate':
         generate(a.first, a.count, a.seed, a.nz, a.batch, a.max_attempts, a.poll, a.out, data=a.data,
                  epochs=a.epochs, min_train=a.min_train, time_budget=a.time_budget)
+    elif a.cmd == 'train':
+        train_only(a.data, a.hidden, a.layers, a.epochs, a.mb, a.lr, a.out)
+    elif a.cmd == 'evaluate':
+        rep = evaluate(a.model, a.test, a.test_seed, a.max_attempts, a.poll)
+        log(json.dumps(rep))
+        with open(a.out, 'w') as f:
+            json.dump(rep, f, indent=1)
     else:
         train_eval(a.data, a.test, a.test_seed, a.hidden, a.layers, a.epochs, a.mb, a.lr, a.max_attempts, a.poll,
                    a.out, a.save_model)
