@@ -188,7 +188,7 @@ FACTOR_VARIANTS = [0, 1, 2, 3, 4]   # register GJ, blocked MFMA (1 or 2 WGs per 
 @gpu
 @pytest.mark.parametrize('variant', FACTOR_VARIANTS)
 @pytest.mark.parametrize('ni,nz', [(30, 60), (93, 150), (150, 12), (7, 3), (16, 2), (40, 3), (17, 5), (112, 9),
-                                   (131, 7)])
+                                   (131, 7), (64, 4), (65, 3), (128, 3), (145, 4)])
 def test_block_factor_solve_against_dense(ni, nz, variant):
     torch = _torch()
     if variant == 3 and ni > 112:

@@ -258,25 +258,24 @@ __device__ __forceinline__ double lhs_diag(double val, int i, int mol, const dou
 }
 
 // one entry of the lhs (mode 1) or of the Jacobian (mode 0): its terms in order
-__device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_ent,
-                                          const vk_jac_term *__restrict__ jac_term, const double *dv, int ni,
-                                          int mode, int mol, double c0, const double *dg0, const double *dg1,
-                                          const double *dgx, int xmode, double *__restrict__ Dl) {
+__device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_ent, const int *__restrict__ jac_tp,
+                                          const double *dv, int ni, int mode, int mol, double c0, const double *dg0,
+                                          const double *dg1, const double *dgx, int xmode, double *__restrict__ Dl) {
     const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
     double J = 0.0;
     if (en.z > en.y) {
-        vk_jac_term t = jac_term[en.y];
-        J = t.coef * dv[t.slot];
+        const int t = jac_tp[en.y];
+        J = vk_jac_coef(t) * dv[vk_jac_slot(t)];
         // terms in order; four loads in flight per round
         for (int u = en.y + 1; u < en.z; u += 4) {
-            const vk_jac_term t0 = jac_term[u];
-            const vk_jac_term t1 = u + 1 < en.z ? jac_term[u + 1] : t0;
-            const vk_jac_term t2 = u + 2 < en.z ? jac_term[u + 2] : t0;
-            const vk_jac_term t3 = u + 3 < en.z ? jac_term[u + 3] : t0;
-            J = J + t0.coef * dv[t0.slot];
-            if (u + 1 < en.z) J = J + t1.coef * dv[t1.slot];
-            if (u + 2 < en.z) J = J + t2.coef * dv[t2.slot];
-            if (u + 3 < en.z) J = J + t3.coef * dv[t3.slot];
+            const int t0 = jac_tp[u];
+            const int t1 = u + 1 < en.z ? jac_tp[u + 1] : t0;
+            const int t2 = u + 2 < en.z ? jac_tp[u + 2] : t0;
+            const int t3 = u + 3 < en.z ? jac_tp[u + 3] : t0;
+            J = J + vk_jac_coef(t0) * dv[vk_jac_slot(t0)];
+            if (u + 1 < en.z) J = J + vk_jac_coef(t1) * dv[vk_jac_slot(t1)];
+            if (u + 2 < en.z) J = J + vk_jac_coef(t2) * dv[vk_jac_slot(t2)];
+            if (u + 3 < en.z) J = J + vk_jac_coef(t3) * dv[vk_jac_slot(t3)];
         }
     }
     if (mode == 0) {
@@ -293,7 +292,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     int ni, int npairs, int nslot, int nent, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
     const int *__restrict__ slot_sp, const int4 *__restrict__ jac_ent, const int *__restrict__ jac_emap, int nheavy,
-    const vk_jac_term *__restrict__ jac_term,
+    const int *__restrict__ jac_tp,
     const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y,
     const double *__restrict__ k, const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
     const double *__restrict__ ysum, const vk_colstate *__restrict__ st, double *__restrict__ D,
@@ -361,7 +360,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     // workgroup forms the layer's shared terms itself).
     const int e0 = threadIdx.x + blockIdx.y * blockDim.x, es = blockDim.x * gridDim.y;
     for (int q = e0; q < nheavy; q += es)
-        jac_entry(q, jac_ent, jac_term, dv, ni, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
+        jac_entry(q, jac_ent, jac_tp, dv, ni, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
 #pragma unroll 2
     for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
@@ -372,13 +371,14 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
             const int u0 = code >> 4, nt = code & 15;
             double J = 0.0;
             if (nt > 0) {
-                vk_jac_term t[VK_JAC_LIGHT];
+                // only the entry's own terms are loaded (all issued before the sums)
+                int t[VK_JAC_LIGHT];
 #pragma unroll
-                for (int m = 0; m < VK_JAC_LIGHT; m++) t[m] = jac_term[u0 + (m < nt ? m : 0)];
-                J = t[0].coef * dv[t[0].slot];
+                for (int m = 0; m < VK_JAC_LIGHT; m++) t[m] = m < nt ? jac_tp[u0 + m] : 0;
+                J = vk_jac_coef(t[0]) * dv[vk_jac_slot(t[0])];
 #pragma unroll
                 for (int m = 1; m < VK_JAC_LIGHT; m++)
-                    if (m < nt) J = J + t[m].coef * dv[t[m].slot];
+                    if (m < nt) J = J + vk_jac_coef(t[m]) * dv[vk_jac_slot(t[m])];
             }
             if (mode == 0) {
                 val = J;
@@ -462,7 +462,7 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     const int gy = ncu > 0 ? std::max(1, std::min(4, 2 * ncu / std::max(cols->L, 1))) : 1;
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L, gy), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        plan->njent, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
-                       plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_term, plan->ms, plan->alpha, a, y, k, M, mode, mol,
+                       plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_tp, plan->ms, plan->alpha, a, y, k, M, mode, mol,
                        dtc, ysum, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;

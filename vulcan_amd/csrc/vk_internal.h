@@ -16,10 +16,13 @@
 
 
 // one term of a Jacobian entry: coef * dv[slot] (one 16-byte load)
-struct vk_jac_term {
-    double coef;
-    int slot, pad;
-};
+// one Jacobian term packed in 32 bits: dv slot << 8 | net stoichiometric
+// coefficient (a small integer, signed 8 bits); the coefficient converts
+// back to the same double, so products and sums are unchanged
+constexpr int VK_JAC_SLOT_MAX = 1 << 23;
+__host__ __device__ inline int vk_jac_pack(int slot, int coef) { return (slot << 8) | (coef & 0xFF); }
+__host__ __device__ inline int vk_jac_slot(int w) { return w >> 8; }
+__host__ __device__ inline double vk_jac_coef(int w) { return (double)((int)((unsigned)w << 24) >> 24); }
 
 struct vk_plan {
     int ni, npairs, nr, natom, nchem, nslot, njnz, njac;
@@ -34,7 +37,7 @@ struct vk_plan {
     int4 *jac_ent;             // (dense index, first term, end of terms, row), longest term lists first
     int *jac_emap;             // dense index -> (first term << 4 | term count) of a short list, -1 zero, -2 long list
     int njheavy;               // entries with more than VK_JAC_LIGHT terms (the first ones of jac_ent)
-    vk_jac_term *jac_term;     // jac_coef / jac_slot interleaved
+    int *jac_tp;               // packed terms (vk_jac_pack) in jac_coef / jac_slot order
     double *jac_coef;
     std::vector<void *> allocs;
 };

@@ -532,9 +532,15 @@ extern "C" int vk_plan_create(const vk_network_desc *n, vk_plan **out) {
         r = upload(v, &p->chem_ent, ce.data(), ce.size());
     }
     if (!r) {
-        std::vector<vk_jac_term> terms((size_t)n->njac);
-        for (size_t q = 0; q < terms.size(); q++) terms[q] = vk_jac_term{n->jac_coef[q], n->jac_slot[q], 0};
-        r = upload(v, &p->jac_term, terms.data(), terms.size());
+        std::vector<int> terms((size_t)n->njac);
+        for (size_t q = 0; q < terms.size() && !r; q++) {
+            const double c = n->jac_coef[q];
+            const int sl = n->jac_slot[q];
+            // the coefficients are net stoichiometric numbers
+            if (c != (double)(int)c || c < -128.0 || c > 127.0 || sl < 0 || sl >= VK_JAC_SLOT_MAX) r = VK_EINVAL;
+            else terms[q] = vk_jac_pack(sl, (int)c);
+        }
+        r = r ? r : upload(v, &p->jac_tp, terms.data(), terms.size());
     }
     if (r) { vk_plan_destroy(p); return r; }
     *out = p;

@@ -52,35 +52,65 @@ __global__ void k_zero_int(int n, int *__restrict__ p) {
 // ---------------------------------------------------------------------------
 // Twisted block solves, 1024-thread workgroups.  Thread (i, h) = (tid % RP,
 // tid / RP) owns species row i and the h-th of SP column chunks of each
-// inverse (RP = 128 or 256 rows, SP = 8 or 4 parts), so one layer's
-// matrix-vector product is SP-way split and the
-// Dinv chunks of the next three layers are in flight while the current layer
-// is processed (the chain over layers is latency bound, not bandwidth bound).
+// inverse (SolveShape below), so one layer's matrix-vector product is
+// SP-way split, and the Dinv chunks of the next layers (SolveDepth) are in
+// flight while the current layer is processed (the chain over layers is
+// latency bound, not bandwidth bound).
 // Both sweeps stay within the column: layer indices run over [0, nz).
 // ---------------------------------------------------------------------------
 constexpr int NTS = 1024;
 
-// rows per part (a power of two >= 16 NB) and column chunk per thread
+// rows per part (64, 128, or the padded block size 16 NB above 128
+// species), parts (16 or 8; 7 or 6 for NB 9 or 10: threads from SP * RP on
+// hold no part) and column chunk per thread (CH <= 27 doubles, so that one
+// layer's chunk stays in registers at 1024 threads; CH <= 16 (NB - 1) + 1,
+// the fewest species of NB blocks, so a chunk always fits in the block)
 template <int NB>
 struct SolveShape {
-    static constexpr int RP = 16 * NB <= 128 ? 128 : 256;
+    static constexpr int RP = 16 * NB <= 64 ? 64 : (16 * NB <= 128 ? 128 : 16 * NB);
     static constexpr int SP = NTS / RP;
     static constexpr int CH = (16 * NB + SP - 1) / SP;
+    static_assert(SP * RP <= NTS && SP * CH >= 16 * NB && CH <= 16 * (NB - 1) + 1, "solve shape");
 };
 
+// Where a thread's chunk of a layer's inverse lies: rows mb .. mb+CH-1 of
+// the transposed inverse (column ib).  A part's chunk starts at m0 = h CH;
+// the last part's is moved back to end at row ni-1 (mb = ni - CH) and its
+// first d = m0 - mb entries, which belong to the part before, are zeroed;
+// rows i >= ni read column ni-1 (their sums are never used) and threads
+// without a part read part 0's chunk.  Every load stays inside the block.
+struct ChunkMap {
+    int voff;   // byte offset of (mb, ib) in the block
+    int d;      // leading entries to zero
+    int mb;     // first row of the chunk
+};
+
+template <int SP, int CH>
+__device__ __forceinline__ ChunkMap chunk_map(int ni, int i, int h) {
+    const int m0 = (h < SP ? h : 0) * CH;
+    const int mb = min(m0, ni - CH);
+    return {(mb * ni + min(i, ni - 1)) * 8, m0 - mb, mb};
+}
+
+// r[q] = Xt[mb + q][ib] (zero for q < d): raw buffer loads with the
+// thread's offset in one register and the uniform row stride q ni in the
+// scalar offset, over the block's byte range
 template <int CH>
-__device__ __forceinline__ void load_chunk(double (&r)[CH], const double *__restrict__ Xt, int ni, int i, int m0) {
+__device__ __forceinline__ void load_chunk(double (&r)[CH], const double *__restrict__ Xt, int ni,
+                                           const ChunkMap &cm) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(Xt), (short)0, ni * ni * 8, 0x00020000);
 #pragma unroll
     for (int q = 0; q < CH; q++) {
-        const int mm = m0 + q;
-        r[q] = (i < ni && mm < ni) ? Xt[(size_t)mm * ni + i] : 0.0;
+        const double x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, cm.voff, q * ni * 8, 0));
+        r[q] = q >= cm.d ? x : 0.0;
     }
 }
 
 // layers in flight per thread: fewer for larger blocks (128 registers per thread at 1024 threads)
 template <int NB>
 struct SolveDepth {
-    static constexpr int value = NB <= 4 ? 4 : (NB <= 5 ? 3 : (NB <= 7 ? 2 : 1));
+    static constexpr int value = NB <= 4 ? 4 : (NB <= 6 ? 3 : (NB <= 8 ? 2 : 1));
 };
 
 template <int DEP, int CH>
@@ -101,7 +131,7 @@ __device__ __forceinline__ double solve_matvec(const double (&cur)[CH], const do
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < CH; q++) acc = fma(cur[q], v[m0 + q], acc);
-    part[h][i] = acc;
+    if (h < SP) part[h][i] = acc;
     __syncthreads();
     double y = part[0][i];
 #pragma unroll
@@ -117,7 +147,8 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
     constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
     const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
     if (st && st[c].done) return;
-    const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
+    const int tid = threadIdx.x, i = tid % RP, h = tid / RP;
+    const ChunkMap cm = chunk_map<SP, CH>(ni, i, h);
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
     // v and part double-buffered by step parity: two barriers per step; the
@@ -132,7 +163,7 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
     double buf[DEP][CH];
 #pragma unroll
     for (int d = 0; d < DEP; d++)
-        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)(l0 + j0 + d * dj) * nn, ni, i, m0);
+        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)(l0 + j0 + d * dj) * nn, ni, cm);
     const bool row = h == 0 && i < ni;
     double zi = 0.0, in_n = 0.0, cp_n = 0.0;
     if (row && nsteps > 0) {
@@ -154,13 +185,13 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
             v[pb][i] = w;
         }
         __syncthreads();
-        const double z = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, m0);
+        const double z = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, cm.mb);
         if (row) {
             tmp[l * ni + i] = z;
             zi = z;
         }
         rotate<DEP, CH>(buf);
-        if (s + DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)(l0 + j0 + dj * (s + DEP)) * nn, ni, i, m0);
+        if (s + DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)(l0 + j0 + dj * (s + DEP)) * nn, ni, cm);
     }
     // park b_m in the unused slot m so the backward kernel never reads `in`
     // (which may alias `out`)
@@ -178,7 +209,8 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
     constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
     const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
     if (st && st[c].done) return;
-    const int tid = threadIdx.x, i = tid % RP, h = tid / RP, m0 = h * CH;
+    const int tid = threadIdx.x, i = tid % RP, h = tid / RP;
+    const ChunkMap cm = chunk_map<SP, CH>(ni, i, h);
     const int l0 = off[c], nz = off[c + 1] - l0, m = nz / 2;
     const size_t nn = (size_t)ni * ni;
     __shared__ double v[2][SP * CH];
@@ -192,7 +224,7 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
     double buf[DEP][CH];
 #pragma unroll
     for (int d = 0; d < DEP; d++)
-        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)((long)lm + (long)d * dj) * nn, ni, i, m0);
+        if (d < nsteps) load_chunk<CH>(buf[d], Dinv + (size_t)((long)lm + (long)d * dj) * nn, ni, cm);
     const bool row = h == 0 && i < ni;
     // x_m = M^-1 (b_m - Lsub_m z_{m-1} - Usup_m w_{m+1}); both sides compute it
     if (row) {
@@ -208,10 +240,10 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
         t_n = tmp[(lm + dj) * ni + i];
     }
     __syncthreads();
-    double xi = solve_matvec<SP, RP, CH>(buf[0], v[0], part[0], i, h, m0);
+    double xi = solve_matvec<SP, RP, CH>(buf[0], v[0], part[0], i, h, cm.mb);
     if (side == 0 && row) out[lm * ni + i] = xi;
     rotate<DEP, CH>(buf);
-    if (DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)DEP * dj) * nn, ni, i, m0);
+    if (DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)DEP * dj) * nn, ni, cm);
     for (int s = 1; s < nsteps; s++) {
         const size_t l = lm + (long)dj * s;
         const int pb = s & 1;
@@ -222,14 +254,14 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
         }
         if (row) v[pb][i] = cp_s * xi;
         __syncthreads();
-        const double y = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, m0);
+        const double y = solve_matvec<SP, RP, CH>(buf[0], v[pb], part[pb], i, h, cm.mb);
         if (row) {
             xi = t_s - y;
             out[l * ni + i] = xi;
         }
         rotate<DEP, CH>(buf);
         if (s + DEP < nsteps)
-            load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)dj * (s + DEP)) * nn, ni, i, m0);
+            load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)dj * (s + DEP)) * nn, ni, cm);
     }
 }
 
