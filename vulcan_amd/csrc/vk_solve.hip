@@ -110,7 +110,7 @@ __device__ __forceinline__ void load_chunk(double (&r)[CH], const double *__rest
 // layers in flight per thread: fewer for larger blocks (128 registers per thread at 1024 threads)
 template <int NB>
 struct SolveDepth {
-    static constexpr int value = NB <= 4 ? 4 : (NB <= 6 ? 3 : (NB <= 8 ? 2 : 1));
+    static constexpr int value = NB <= 4 ? 4 : (NB <= 5 ? 3 : (NB <= 8 ? 2 : 1));
 };
 
 template <int DEP, int CH>
