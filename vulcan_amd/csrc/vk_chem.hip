@@ -221,7 +221,16 @@ __global__ void __launch_bounds__(BLK) k_rhs(
             if (lo < hi) {
                 int2 t = chem_ent[lo];                 // (pair, sign)
                 chem = t.y * v[t.x];
-                for (int e = lo + 1; e < hi; e++) {
+                // the terms in order, eight entries loaded ahead of their sums
+                int e = lo + 1;
+                for (; e + 8 <= hi; e += 8) {
+                    int2 tt[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) tt[u] = chem_ent[e + u];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) chem = chem + tt[u].y * v[tt[u].x];
+                }
+                for (; e < hi; e++) {
                     t = chem_ent[e];
                     chem = chem + t.y * v[t.x];
                 }
@@ -312,12 +321,37 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     double *Dl = D + (size_t)l * nn;
     const double *kl = k + (size_t)l * 2 * npairs;
     const double m = M[l];
-    for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
-        int p = slot_pair[s], sp = slot_sp[s];
-        bool a1, a2;
-        double dr = dside(((const int4 *)reac)[p], kl[2 * p], yl, m, sp, a1);
-        double dp = dside(((const int4 *)prod)[p], kl[2 * p + 1], yl, m, sp, a2);
-        dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
+    // slot derivatives: JS slots per thread per round, every load of a round
+    // (slot -> pair -> the pair's sides and rates, three loads deep) issued
+    // before its arithmetic
+    constexpr int JS = 4;
+    for (int s0 = threadIdx.x; s0 < nslot; s0 += JS * blockDim.x) {
+        int p[JS], sp[JS];
+#pragma unroll
+        for (int u = 0; u < JS; u++) {
+            const int s = s0 + u * blockDim.x;
+            p[u] = s < nslot ? slot_pair[s] : 0;
+            sp[u] = s < nslot ? slot_sp[s] : 0;
+        }
+        int4 rt[JS], pt[JS];
+        double kf[JS], kb[JS];
+#pragma unroll
+        for (int u = 0; u < JS; u++) {
+            rt[u] = ((const int4 *)reac)[p[u]];
+            pt[u] = ((const int4 *)prod)[p[u]];
+            kf[u] = kl[2 * p[u]];
+            kb[u] = kl[2 * p[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < JS; u++) {
+            const int s = s0 + u * blockDim.x;
+            if (s < nslot) {
+                bool a1, a2;
+                const double dr = dside(rt[u], kf[u], yl, m, sp[u], a1);
+                const double dp = dside(pt[u], kb[u], yl, m, sp[u], a2);
+                dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
+            }
+        }
     }
     double c0 = 0.0;
     int xmode = 0;
@@ -361,7 +395,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     const int e0 = threadIdx.x + blockIdx.y * blockDim.x, es = blockDim.x * gridDim.y;
     for (int q = e0; q < nheavy; q += es)
         jac_entry(q, jac_ent, jac_tp, dv, ni, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
-#pragma unroll 2
+#pragma unroll 4
     for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
         const int code = jac_emap[e];
