@@ -103,14 +103,15 @@ __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *
         if (s > 0) rmax = fmax(rmax, d / s);
         sol[i] = s;
     }
-    red[threadIdx.x] = rmax;
+    // maximum over the workgroup (exact in any order): within each wave by
+    // lane exchanges, then over the waves
+    for (int o = 32; o > 0; o >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        part[(size_t)l * NP + VKP_DELTA] = red[0];
+        double r = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmax(r, red[q]);
+        part[(size_t)l * NP + VKP_DELTA] = r;
         ysum = pw_sum(sol, ni);
     }
     __syncthreads();
@@ -141,7 +142,10 @@ __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *
     if (threadIdx.x == 0) {
         double s1 = 0.0, s2 = 0.0;
         int nf = 0;
-        for (int t = 0; t < (int)blockDim.x; t++) { s1 += sm[t]; s2 += ng[t]; nf |= negf[t]; }
+        // threads from ni on hold zero partials (adding +0.0 to a sum that
+        // starts at +0.0 changes nothing)
+        const int nt = min((int)blockDim.x, ni);
+        for (int t = 0; t < nt; t++) { s1 += sm[t]; s2 += ng[t]; nf |= negf[t]; }
         part[(size_t)l * NP + VKP_NEG] = nf;
         part[(size_t)l * NP + VKP_SMALL] = s1;
         part[(size_t)l * NP + VKP_NEGY] = s2;
