@@ -34,6 +34,7 @@ extern "C" {
 #define VK_EINVAL   (-1001)   /* bad argument / shape            */
 #define VK_ENOMEM   (-1002)   /* device allocation failed        */
 #define VK_ETOOBIG  (-1003)   /* ni beyond the compiled maximum  */
+#define VK_ENOTCHECKED (-1004) /* query of the bounds-checked build on the normal one */
 
 #define VK_NI_LIMIT 160           /* largest ni the kernels are compiled for */
 #define VK_SPECIES_WORDS 5        /* ceil(VK_NI_LIMIT / 32): one bit per species */
@@ -415,6 +416,16 @@ int vk_mlp_to_bf16(long n, const float *d_x, void *d_y, void *stream);
  * weights; refreshes the BF16 copy d_wb when not NULL */
 int vk_mlp_adam(long n, float *d_w, const float *d_g, float *d_m, float *d_v, float lr, float b1,
                 float b2, float eps, int step, float wd, void *d_wb, void *stream);
+
+/* ---- debug aid -----------------------------------------------------------*/
+/* No counterpart in the reference (a debug aid for this library).  In the
+ * bounds-checked build (libvulcan_amd_checked.so, the chemistry kernels
+ * compiled with VK_CHECKED) every table index the rate, right-hand-side and
+ * assembly kernels gather through is checked against its array; *site_out
+ * receives 0 if none was out of range since the last reset, else the site
+ * of the first (enum VkCheckSite in vk_chem.hip); reset != 0 clears it.
+ * The normal build returns VK_ENOTCHECKED. */
+int vk_debug_checks(int *site_out, int reset);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libvulcan_amd.so')
+# the same library with the chemistry kernels' table indexes bounds-checked
+# (vk_debug_checks); a debugging aid, not loaded by the product path
+CHECKED_LIB_PATH = os.path.join(_HERE, 'libvulcan_amd_checked.so')
 
 c_int_p = C.POINTER(C.c_int)
 c_double_p = C.POINTER(C.c_double)
@@ -95,6 +98,7 @@ _SIGS = {
     'vk_integrator_set_photo': [VP, VP, C.c_int, C.c_int],
     'vk_update_mu_dz': [VP, VP, C.POINTER(Atm), C.POINTER(Geom), VP, VP, C.c_int, VP],
     'vk_update_phi_esc': [VP, VP, C.POINTER(Atm), VP, VP],
+    'vk_debug_checks': [C.POINTER(C.c_int), C.c_int],
     'vk_atom_sums': [VP, VP, VP, VP, VP],
     'vk_integrator_set_timing': [VP, C.c_int],
     'vk_integrator_set_graph': [VP, C.c_int],

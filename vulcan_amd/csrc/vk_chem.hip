@@ -9,6 +9,33 @@
 
 using namespace vk;
 
+// Bounds-checked build (compiled with VK_CHECKED, libvulcan_amd_checked.so):
+// every index the gather/scatter kernels read from their tables is checked
+// against the length of the array it indexes; the first index out of range
+// records its site in vk_check_word and the access uses index 0 instead,
+// so nothing is read or written outside its array.  The normal build
+// compiles the checks away.
+enum VkCheckSite {
+    VK_SITE_SPECIES = 1,   // species of a reaction side (rates of pairs, slot derivatives)
+    VK_SITE_PAIR = 2,      // pair of a production/loss entry of a species
+    VK_SITE_CHEM = 3,      // production/loss entry of a species
+    VK_SITE_SLOTPAIR = 4,  // pair of a derivative slot
+    VK_SITE_TERM = 5,      // Jacobian term of an entry
+    VK_SITE_SLOT = 6,      // derivative slot of a Jacobian term
+    VK_SITE_DENSE = 7,     // dense index of a long Jacobian entry
+};
+#ifdef VK_CHECKED
+static __device__ int vk_check_word;
+__device__ __forceinline__ int vk_ix(int i, int n, int site) {
+    if ((unsigned)i < (unsigned)n) return i;
+    atomicCAS(&vk_check_word, 0, site);
+    return 0;
+}
+#define VK_IX(i, n, site) vk_ix((i), (n), (site))
+#else
+#define VK_IX(i, n, site) (i)
+#endif
+
 namespace {
 
 constexpr int BLK = 256;
@@ -85,11 +112,21 @@ __global__ void k_rates(int L, int npairs, int ni, const int *__restrict__ kind,
         int nre = 0, npr = 0;
         for (int q = 0; q < 4; q++) {
             int t = reac[4 * p + q];
-            if (t >= 0) { acc = first ? -1 * g[t] : acc - 1 * g[t]; first = false; nre++; }
+            if (t >= 0) {
+                t = VK_IX(t, ni, VK_SITE_SPECIES);
+                acc = first ? -1 * g[t] : acc - 1 * g[t];
+                first = false;
+                nre++;
+            }
         }
         for (int q = 0; q < 4; q++) {
             int t = prod[4 * p + q];
-            if (t >= 0) { acc = first ? 1 * g[t] : acc + 1 * g[t]; first = false; npr++; }
+            if (t >= 0) {
+                t = VK_IX(t, ni, VK_SITE_SPECIES);
+                acc = first ? 1 * g[t] : acc + 1 * g[t];
+                first = false;
+                npr++;
+            }
         }
         double K = exp(-(acc));
         int dn = nre - npr;
@@ -128,26 +165,26 @@ __device__ inline LayerCtx layer_ctx(const int *off, const int *layer_col, int l
 
 // v_p = k_f * prod(reac) - k_r * prod(prod), left to right (make_chem_funs.py:293-315);
 // the four terms of a side are one 16-byte load
-__device__ inline double side_product(int4 t4, double kcol, const double *yl, double m) {
+__device__ inline double side_product(int4 t4, double kcol, const double *yl, double m, int ni) {
     const int t[4] = {t4.x, t4.y, t4.z, t4.w};
     double a = kcol;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (t[q] == -2) continue;
-        a = a * (t[q] == -1 ? m : yl[t[q]]);
+        a = a * (t[q] == -1 ? m : yl[VK_IX(t[q], ni, VK_SITE_SPECIES)]);
     }
     return a;
 }
 
 __device__ inline double pair_rate(const int4 *reac, const int4 *prod, const double *kl, const double *yl,
-                                   double m, int p) {
-    const double a = side_product(reac[p], kl[2 * p], yl, m);
-    const double b = side_product(prod[p], kl[2 * p + 1], yl, m);
+                                   double m, int p, int ni) {
+    const double a = side_product(reac[p], kl[2 * p], yl, m, ni);
+    const double b = side_product(prod[p], kl[2 * p + 1], yl, m, ni);
     return a - b;
 }
 
 // d/dy_sp of kcol * prod(terms)
-__device__ inline double dside(int4 t4, double kcol, const double *yl, double m, int sp, bool &any) {
+__device__ inline double dside(int4 t4, double kcol, const double *yl, double m, int sp, bool &any, int ni) {
     const int t[4] = {t4.x, t4.y, t4.z, t4.w};
     double tot = 0.0;
     any = false;
@@ -158,7 +195,7 @@ __device__ inline double dside(int4 t4, double kcol, const double *yl, double m,
 #pragma unroll
         for (int q2 = 0; q2 < 4; q2++) {
             if (q2 == q || t[q2] == -2) continue;
-            acc = acc * (t[q2] == -1 ? m : yl[t[q2]]);
+            acc = acc * (t[q2] == -1 ? m : yl[VK_IX(t[q2], ni, VK_SITE_SPECIES)]);
         }
         tot = any ? tot + acc : acc;
         any = true;
@@ -186,7 +223,7 @@ __device__ inline void ysum_around(const double *ysum, const LayerCtx &x, double
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(BLK) k_rhs(
-    int ni, int npairs, const int *__restrict__ off, const int *__restrict__ layer_col,
+    int ni, int npairs, int nchem, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ chem_ptr,
     const int2 *__restrict__ chem_ent, const double *__restrict__ ms,
     const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
@@ -208,7 +245,7 @@ __global__ void __launch_bounds__(BLK) k_rhs(
         const double *kl = k + (size_t)l * 2 * npairs;
         double m = M[l];
         for (int p = threadIdx.x; p < npairs; p += blockDim.x)
-            v[p] = pair_rate((const int4 *)reac, (const int4 *)prod, kl, yl, m, p);
+            v[p] = pair_rate((const int4 *)reac, (const int4 *)prod, kl, yl, m, p, ni);
         __syncthreads();
     }
     ColAtm ca = col_atm(atm, x.c, x.off, x.nz, ni);
@@ -219,20 +256,20 @@ __global__ void __launch_bounds__(BLK) k_rhs(
         if (mode & 1) {
             const int lo = chem_ptr[i], hi = chem_ptr[i + 1];
             if (lo < hi) {
-                int2 t = chem_ent[lo];                 // (pair, sign)
-                chem = t.y * v[t.x];
+                int2 t = chem_ent[VK_IX(lo, nchem, VK_SITE_CHEM)];   // (pair, sign)
+                chem = t.y * v[VK_IX(t.x, npairs, VK_SITE_PAIR)];
                 // the terms in order, eight entries loaded ahead of their sums
                 int e = lo + 1;
                 for (; e + 8 <= hi; e += 8) {
                     int2 tt[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) tt[u] = chem_ent[e + u];
+                    for (int u = 0; u < 8; u++) tt[u] = chem_ent[VK_IX(e + u, nchem, VK_SITE_CHEM)];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) chem = chem + tt[u].y * v[tt[u].x];
+                    for (int u = 0; u < 8; u++) chem = chem + tt[u].y * v[VK_IX(tt[u].x, npairs, VK_SITE_PAIR)];
                 }
                 for (; e < hi; e++) {
-                    t = chem_ent[e];
-                    chem = chem + t.y * v[t.x];
+                    t = chem_ent[VK_IX(e, nchem, VK_SITE_CHEM)];
+                    chem = chem + t.y * v[VK_IX(t.x, npairs, VK_SITE_PAIR)];
                 }
             }
         }
@@ -268,37 +305,39 @@ __device__ __forceinline__ double lhs_diag(double val, int i, int mol, const dou
 
 // one entry of the lhs (mode 1) or of the Jacobian (mode 0): its terms in order
 __device__ __forceinline__ void jac_entry(int q, const int4 *__restrict__ jac_ent, const int *__restrict__ jac_tp,
-                                          const double *dv, int ni, int mode, int mol, double c0, const double *dg0,
-                                          const double *dg1, const double *dgx, int xmode, double *__restrict__ Dl) {
+                                          const double *dv, int ni, int njac, int nslot, int mode, int mol, double c0,
+                                          const double *dg0, const double *dg1, const double *dgx, int xmode,
+                                          double *__restrict__ Dl) {
     const int4 en = jac_ent[q];           // (dense index, first term, end of terms, row)
     double J = 0.0;
     if (en.z > en.y) {
-        const int t = jac_tp[en.y];
-        J = vk_jac_coef(t) * dv[vk_jac_slot(t)];
+        const int t = jac_tp[VK_IX(en.y, njac, VK_SITE_TERM)];
+        J = vk_jac_coef(t) * dv[VK_IX(vk_jac_slot(t), nslot, VK_SITE_SLOT)];
         // terms in order; four loads in flight per round
         for (int u = en.y + 1; u < en.z; u += 4) {
-            const int t0 = jac_tp[u];
-            const int t1 = u + 1 < en.z ? jac_tp[u + 1] : t0;
-            const int t2 = u + 2 < en.z ? jac_tp[u + 2] : t0;
-            const int t3 = u + 3 < en.z ? jac_tp[u + 3] : t0;
-            J = J + vk_jac_coef(t0) * dv[vk_jac_slot(t0)];
-            if (u + 1 < en.z) J = J + vk_jac_coef(t1) * dv[vk_jac_slot(t1)];
-            if (u + 2 < en.z) J = J + vk_jac_coef(t2) * dv[vk_jac_slot(t2)];
-            if (u + 3 < en.z) J = J + vk_jac_coef(t3) * dv[vk_jac_slot(t3)];
+            const int t0 = jac_tp[VK_IX(u, njac, VK_SITE_TERM)];
+            const int t1 = u + 1 < en.z ? jac_tp[VK_IX(u + 1, njac, VK_SITE_TERM)] : t0;
+            const int t2 = u + 2 < en.z ? jac_tp[VK_IX(u + 2, njac, VK_SITE_TERM)] : t0;
+            const int t3 = u + 3 < en.z ? jac_tp[VK_IX(u + 3, njac, VK_SITE_TERM)] : t0;
+            J = J + vk_jac_coef(t0) * dv[VK_IX(vk_jac_slot(t0), nslot, VK_SITE_SLOT)];
+            if (u + 1 < en.z) J = J + vk_jac_coef(t1) * dv[VK_IX(vk_jac_slot(t1), nslot, VK_SITE_SLOT)];
+            if (u + 2 < en.z) J = J + vk_jac_coef(t2) * dv[VK_IX(vk_jac_slot(t2), nslot, VK_SITE_SLOT)];
+            if (u + 3 < en.z) J = J + vk_jac_coef(t3) * dv[VK_IX(vk_jac_slot(t3), nslot, VK_SITE_SLOT)];
         }
     }
+    const int ex = VK_IX(en.x, ni * ni, VK_SITE_DENSE);
     if (mode == 0) {
-        Dl[en.x] = J;
+        Dl[ex] = J;
         return;
     }
     const int i = en.w;
     double val = -J;
-    if (en.x == i * ni + i) val = lhs_diag(c0 + val, i, mol, dg0, dg1, dgx, xmode);
-    Dl[en.x] = val;
+    if (ex == i * ni + i) val = lhs_diag(c0 + val, i, mol, dg0, dg1, dgx, xmode);
+    Dl[ex] = val;
 }
 
 __global__ void __launch_bounds__(BLK) k_jac_lhs(
-    int ni, int npairs, int nslot, int nent, const int *__restrict__ off, const int *__restrict__ layer_col,
+    int ni, int npairs, int nslot, int nent, int njac, const int *__restrict__ off, const int *__restrict__ layer_col,
     const int *__restrict__ reac, const int *__restrict__ prod, const int *__restrict__ slot_pair,
     const int *__restrict__ slot_sp, const int4 *__restrict__ jac_ent, const int *__restrict__ jac_emap, int nheavy,
     const int *__restrict__ jac_tp,
@@ -330,7 +369,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
 #pragma unroll
         for (int u = 0; u < JS; u++) {
             const int s = s0 + u * blockDim.x;
-            p[u] = s < nslot ? slot_pair[s] : 0;
+            p[u] = s < nslot ? VK_IX(slot_pair[s], npairs, VK_SITE_SLOTPAIR) : 0;
             sp[u] = s < nslot ? slot_sp[s] : 0;
         }
         int4 rt[JS], pt[JS];
@@ -347,8 +386,8 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
             const int s = s0 + u * blockDim.x;
             if (s < nslot) {
                 bool a1, a2;
-                const double dr = dside(rt[u], kf[u], yl, m, sp[u], a1);
-                const double dp = dside(pt[u], kb[u], yl, m, sp[u], a2);
+                const double dr = dside(rt[u], kf[u], yl, m, sp[u], a1, ni);
+                const double dp = dside(pt[u], kb[u], yl, m, sp[u], a2, ni);
                 dv[s] = (a1 ? dr : 0.0) - (a2 ? dp : 0.0);
             }
         }
@@ -394,7 +433,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     // workgroup forms the layer's shared terms itself).
     const int e0 = threadIdx.x + blockIdx.y * blockDim.x, es = blockDim.x * gridDim.y;
     for (int q = e0; q < nheavy; q += es)
-        jac_entry(q, jac_ent, jac_tp, dv, ni, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
+        jac_entry(q, jac_ent, jac_tp, dv, ni, njac, nslot, mode, mol, c0, dg0, dg1, dgx, xmode, Dl);
 #pragma unroll 4
     for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
@@ -408,11 +447,11 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
                 // only the entry's own terms are loaded (all issued before the sums)
                 int t[VK_JAC_LIGHT];
 #pragma unroll
-                for (int m = 0; m < VK_JAC_LIGHT; m++) t[m] = m < nt ? jac_tp[u0 + m] : 0;
-                J = vk_jac_coef(t[0]) * dv[vk_jac_slot(t[0])];
+                for (int m = 0; m < VK_JAC_LIGHT; m++) t[m] = m < nt ? jac_tp[VK_IX(u0 + m, njac, VK_SITE_TERM)] : 0;
+                J = vk_jac_coef(t[0]) * dv[VK_IX(vk_jac_slot(t[0]), nslot, VK_SITE_SLOT)];
 #pragma unroll
                 for (int m = 1; m < VK_JAC_LIGHT; m++)
-                    if (m < nt) J = J + vk_jac_coef(t[m]) * dv[vk_jac_slot(t[m])];
+                    if (m < nt) J = J + vk_jac_coef(t[m]) * dv[VK_IX(vk_jac_slot(t[m]), nslot, VK_SITE_SLOT)];
             }
             if (mode == 0) {
                 val = J;
@@ -468,7 +507,8 @@ int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm
         int r = launch_ysum(cols, plan->ni, y, ysum, s);
         if (r) return r;
     }
-    hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, cols->d_off,
+    hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, plan->nchem,
+                       cols->d_off,
                        cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_ent, plan->ms,
                        plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, ysum, st, out);
     VK_CHECK(hipGetLastError());
@@ -495,7 +535,7 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     }();
     const int gy = ncu > 0 ? std::max(1, std::min(4, 2 * ncu / std::max(cols->L, 1))) : 1;
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L, gy), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
-                       plan->njent, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
+                       plan->njent, plan->njac, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
                        plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_tp, plan->ms, plan->alpha, a, y, k, M, mode, mol,
                        dtc, ysum, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
@@ -552,4 +592,21 @@ extern "C" int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     if (ys.alloc(cols->L)) return VK_ENOMEM;
     return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, ys.p, d_D, d_Lsub, d_Usup,
                              S(stream));
+}
+
+extern "C" int vk_debug_checks(int *site_out, int reset) {
+    if (!site_out) return VK_EINVAL;
+#ifdef VK_CHECKED
+    int v = 0;
+    VK_CHECK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(vk_check_word), sizeof(int)));
+    if (reset) {
+        const int z = 0;
+        VK_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(vk_check_word), &z, sizeof(int)));
+    }
+    *site_out = v;
+    return 0;
+#else
+    (void)reset;
+    return VK_ENOTCHECKED;
+#endif
 }
