@@ -515,15 +515,19 @@ int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm
     return 0;
 }
 
+// ysum_ready: the caller's earlier launch on s has left the layer sums of y in ysum
 int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
                       const double *k, const double *M, int mode, int mol, const double *dtc,
-                      const vk_colstate *st, double *ysum, double *D, double *Lsub, double *Usup, hipStream_t s) {
+                      const vk_colstate *st, double *ysum, double *D, double *Lsub, double *Usup, hipStream_t s,
+                      bool ysum_ready = false) {
     vk_atm a{};
     if (atm) a = *atm;
     if (mode == 1) {
         if (!ysum) return VK_EINVAL;
-        int r = launch_ysum(cols, plan->ni, y, ysum, s);
-        if (r) return r;
+        if (!ysum_ready) {
+            int r = launch_ysum(cols, plan->ni, y, ysum, s);
+            if (r) return r;
+        }
     }
     // fewer layers than twice the CUs: split each layer's entries over up to 4 workgroups
     static const int ncu = [] {

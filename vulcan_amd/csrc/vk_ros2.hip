@@ -15,9 +15,9 @@ int vk_launch_rhs(const vk_plan *, const vk_columns *, const vk_atm *, const dou
                   const double *, const double *, const vk_colstate *, double *, double *, hipStream_t);
 int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *,
                       const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
-                      double *, hipStream_t);
+                      double *, hipStream_t, bool ysum_ready = false);
 int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
-                     int *, const vk_colstate *, const vk_bcr_ws *, hipStream_t);
+                     int *, const vk_colstate *, const vk_bcr_ws *, hipStream_t, bool info_zeroed = false);
 int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
                             const double *, const double *, double *, double *, double *, int, const vk_colstate *, const vk_bcr_ws *,
                             hipStream_t);
@@ -54,9 +54,15 @@ __device__ inline void layer_of(const int *off, const int *layer_col, int l, int
     nz = off[c + 1] - o;
 }
 
-__global__ void k_gather_dt(int n, const vk_colstate *__restrict__ st, double *__restrict__ dt) {
+// dt of every column, gathered contiguously; also clears the column's
+// zero-pivot record before the attempt's factorisation
+__global__ void k_gather_dt(int n, const vk_colstate *__restrict__ st, double *__restrict__ dt,
+                            int *__restrict__ info) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n) dt[c] = st[c].dt;
+    if (c < n) {
+        dt[c] = st[c].dt;
+        if (info) info[c] = 0;
+    }
 }
 
 // yk2 = y + k1/r  (op.py:2917)
@@ -793,14 +799,15 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     const size_t L = cl->L;
     // dt per column lives in the state struct; gather it contiguously
     double *dt = it->colscr;
-    hipLaunchKernelGGL(k_gather_dt, dim3((cl->ncols + 127) / 128), dim3(128), 0, s, cl->ncols, st, dt);
+    hipLaunchKernelGGL(k_gather_dt, dim3((cl->ncols + 127) / 128), dim3(128), 0, s, cl->ncols, st, dt, it->info);
     int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->ysum, it->b1, s);
     if (r) return r;
+    // the right-hand side above has left the layer sums of y in it->ysum
     r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->ysum, it->D, it->Lsub, it->Usup,
-                          s);
+                          s, true);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
-    r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, &it->bcr, s);
+    r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, &it->bcr, s, true);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev1, s));
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b1, it->x1, it->work, it->ztmp, prm->refine,

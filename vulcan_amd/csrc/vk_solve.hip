@@ -344,15 +344,16 @@ __global__ void k_axpy(int n, int ni, const int *__restrict__ layer_col, const d
 }  // namespace
 
 
+// info_zeroed: the caller has cleared info in an earlier launch on s
 int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *D, const double *Lsub,
                      const double *Usup, double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws,
-                     hipStream_t s) {
+                     hipStream_t s, bool info_zeroed = false) {
     const int ni = plan->ni;
     const int n = (ni + 15) / 16;
     const int nc = cols->ncols;
     const int v = plan->factor_variant;
     if (n < 1 || n > 10) return VK_ETOOBIG;
-    if (info) hipLaunchKernelGGL(k_zero_int, dim3((nc + 255) / 256), dim3(256), 0, s, nc, info);
+    if (info && !info_zeroed) hipLaunchKernelGGL(k_zero_int, dim3((nc + 255) / 256), dim3(256), 0, s, nc, info);
     int r;
     switch (v) {
     case VK_FACTOR_BCR:
