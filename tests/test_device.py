@@ -42,6 +42,26 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_plan_refuses_networks_beyond_the_compiled_limits():
+    """Refused before anything touches the device: more species than the
+    kernels are compiled for (VK_NI_LIMIT = 160), more pair rates or
+    derivative slots than a layer's 64 KiB of LDS holds, empty networks."""
+    lib = _lib.load()
+
+    def create(**kw):
+        d = _lib.NetworkDesc()
+        d.ni, d.npairs, d.natom = 10, 10, 1
+        for k, v in kw.items():
+            setattr(d, k, v)
+        h = C.c_void_p()
+        return lib.vk_plan_create(C.byref(d), C.byref(h))
+    assert create(ni=161) == -1003          # VK_ETOOBIG
+    assert create(npairs=9000) == -1003
+    assert create(nslot=9000) == -1003
+    assert create(ni=0) == -1001            # VK_EINVAL
+    assert create(npairs=0) == -1001
+
+
 gpu = pytest.mark.gpu
 
 
@@ -188,7 +208,7 @@ FACTOR_VARIANTS = [0, 1, 2, 3, 4]   # register GJ, blocked MFMA (1 or 2 WGs per 
 @gpu
 @pytest.mark.parametrize('variant', FACTOR_VARIANTS)
 @pytest.mark.parametrize('ni,nz', [(30, 60), (93, 150), (150, 12), (7, 3), (16, 2), (40, 3), (17, 5), (112, 9),
-                                   (131, 7), (64, 4), (65, 3), (128, 3), (145, 4)])
+                                   (131, 7), (64, 4), (65, 3), (128, 3), (145, 4), (160, 3)])
 def test_block_factor_solve_against_dense(ni, nz, variant):
     torch = _torch()
     if variant == 3 and ni > 112:
