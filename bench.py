@@ -85,12 +85,17 @@ def _sync():
 
 
 def run_steps(batch, nsteps, col=0):
-    """Attempts until column `col` has taken `nsteps` more accepted steps."""
+    """Attempts until column `col` has taken `nsteps` more accepted steps.
+    An attempt accepts at most one step, so each call launches as many
+    attempts as steps remain (the column never takes more than `nsteps`) and
+    the host reads the step count once per call, not after every attempt."""
     start = int(batch.host_state()['count'][col])
-    attempts = 0
-    while int(batch.host_state()['count'][col]) - start < nsteps:
-        batch.integrate(1, use_conv=False)
-        attempts += 1
+    attempts, done = 0, 0
+    while done < nsteps:
+        n = nsteps - done
+        batch.integrate(n, use_conv=False)
+        attempts += n
+        done = int(batch.host_state()['count'][col]) - start
     return attempts
 
 
