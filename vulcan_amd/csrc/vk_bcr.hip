@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *_
                                                     const int2 *__restrict__ top, const double *__restrict__ Dinv,
                                                     const double *__restrict__ XB, const double *__restrict__ XC,
                                                     const double *__restrict__ r, double *__restrict__ x,
-                                                    const vk_colstate *__restrict__ st) {
+                                                    const vk_colstate *__restrict__ st, double *acc) {
     const int q = blockIdx.x;
     if (q >= n) return;
     int k, left = -1, right = -1, c;
@@ -304,7 +304,11 @@ __global__ void __launch_bounds__(BCR_ST) k_bcr_bwd(int ni, int n, const int4 *_
         pp[h][a] = v;
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < ni; a += BCR_ST) x[(size_t)k * ni + a] = pp[0][a] + pp[1][a];
+    for (int a = threadIdx.x; a < ni; a += BCR_ST) {
+        const double t = pp[0][a] + pp[1][a];
+        x[(size_t)k * ni + a] = t;
+        if (acc) acc[(size_t)k * ni + a] = acc[(size_t)k * ni + a] + t;
+    }
 }
 
 // Solve sweeps for ni <= 128: each block's products are split over row
@@ -378,7 +382,7 @@ __global__ void __launch_bounds__(MV_NT) k_bcr_bwd_s(int ni, int n, const int4 *
                                                      const int2 *__restrict__ top, const double *__restrict__ Dinv,
                                                      const double *__restrict__ XB, const double *__restrict__ XC,
                                                      const double *__restrict__ r, double *__restrict__ x,
-                                                     const vk_colstate *__restrict__ st) {
+                                                     const vk_colstate *__restrict__ st, double *acc) {
     const int q = blockIdx.x, a0 = MV_RG * blockIdx.y;
     if (q >= n || a0 >= ni) return;
     int k, left = -1, right = -1, c;
@@ -406,6 +410,7 @@ __global__ void __launch_bounds__(MV_NT) k_bcr_bwd_s(int ni, int n, const int4 *
 #pragma unroll
         for (int p2 = 0; p2 < MV_SP; p2++) t += pp[p2][threadIdx.x];
         x[(size_t)k * ni + a] = t;
+        if (acc) acc[(size_t)k * ni + a] = acc[(size_t)k * ni + a] + t;
     }
 }
 
@@ -492,7 +497,7 @@ int vk_launch_bcr_factor(int nb, int ni, const vk_columns *cols, const double *D
 
 int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, const double *Lsub, const double *Usup,
                         const double *in, double *out, double *tmp, const vk_colstate *st, const vk_bcr_ws *ws,
-                        hipStream_t s) {
+                        double *acc, hipStream_t s) {
     const vk_bcr_sched &b = cols->bcr;
     const long n = (long)cols->L * ni;
     // ni <= 128: the row-group sweeps (several workgroups per block); their
@@ -513,12 +518,12 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
                                st);
     }
     hipLaunchKernelGGL(bwd, dim3(b.ntop, gy), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, Dinv,
-                       ws->XB, ws->XC, tmp, out, st);
+                       ws->XB, ws->XC, tmp, out, st, acc);
     for (int lev = b.nlev - 1; lev >= 0; lev--) {
         const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
         if (ne > 0)
             hipLaunchKernelGGL(bwd, dim3(ne, gy), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, Dinv,
-                               ws->XB, ws->XC, tmp, out, st);
+                               ws->XB, ws->XC, tmp, out, st, acc);
     }
     VK_CHECK(hipGetLastError());
     return 0;

@@ -89,7 +89,6 @@ struct vk_integrator {
     double *b1, *x1, *b2, *x2, *work;   // right-hand sides and solutions of the two stages
     double *ztmp;                       // forward-sweep scratch of the twisted solve
     double *yk2, *part;
-    double *ysum;                       // layer totals for the diffusion terms
     vk_bcr_ws bcr;                      // block cyclic reduction workspace (allocated when selected)
     int *info;
     double *yhist;             // [history][L][ni]
@@ -122,6 +121,7 @@ int vk_launch_factor_wave(int nb, int ni, int ncols, const int *off, const doubl
 int vk_launch_bcr_factor(int nb, int ni, const vk_columns *cols, const double *D, const double *Lsub,
                          const double *Usup, double *Dinv, int *info, const vk_colstate *st, const vk_bcr_ws *ws,
                          hipStream_t s);
+// acc (nullable): also add the solution into acc, row by row as it is written
 int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, const double *Lsub, const double *Usup,
                         const double *in, double *out, double *tmp, const vk_colstate *st, const vk_bcr_ws *ws,
-                        hipStream_t s);
+                        double *acc, hipStream_t s);

@@ -205,7 +205,7 @@ template <int NB>
 __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restrict__ off,
                                                      const double *__restrict__ Dinv, const double *__restrict__ Lsub,
                                                      const double *__restrict__ Usup, const double *tmp, double *out,
-                                                     const vk_colstate *__restrict__ st) {
+                                                     const vk_colstate *__restrict__ st, double *acc) {
     constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
     const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
     if (st && st[c].done) return;
@@ -241,7 +241,10 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
     }
     __syncthreads();
     double xi = solve_matvec<SP, RP, CH>(buf[0], v[0], part[0], i, h, cm.mb);
-    if (side == 0 && row) out[lm * ni + i] = xi;
+    if (side == 0 && row) {
+        out[lm * ni + i] = xi;
+        if (acc) acc[lm * ni + i] = acc[lm * ni + i] + xi;
+    }
     rotate<DEP, CH>(buf);
     if (DEP < nsteps) load_chunk<CH>(buf[DEP - 1], Dinv + (size_t)((long)lm + (long)DEP * dj) * nn, ni, cm);
     for (int s = 1; s < nsteps; s++) {
@@ -258,6 +261,7 @@ __global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restric
         if (row) {
             xi = t_s - y;
             out[l * ni + i] = xi;
+            if (acc) acc[l * ni + i] = acc[l * ni + i] + xi;
         }
         rotate<DEP, CH>(buf);
         if (s + DEP < nsteps)
@@ -333,13 +337,6 @@ __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__
     }
 }
 
-__global__ void k_axpy(int n, int ni, const int *__restrict__ layer_col, const double *__restrict__ dx,
-                       double *__restrict__ x, const vk_colstate *__restrict__ st) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    if (st && st[layer_col[idx / ni]].done) return;
-    x[idx] = x[idx] + dx[idx];
-}
 
 }  // namespace
 
@@ -383,32 +380,34 @@ int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *
 namespace {
 template <int NB>
 void launch_solve_n(int ni, int ncols, const int *off, const double *Dinv, const double *Lsub, const double *Usup,
-                    const double *in, double *out, double *tmp, const vk_colstate *st, hipStream_t s) {
+                    const double *in, double *out, double *tmp, const vk_colstate *st, double *acc, hipStream_t s) {
     hipLaunchKernelGGL((k_solve_fwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, in, tmp, st);
-    hipLaunchKernelGGL((k_solve_bwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, tmp, out, st);
+    hipLaunchKernelGGL((k_solve_bwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, tmp, out, st,
+                       acc);
 }
 }  // namespace
 
+// acc (nullable): also add the solution into acc, row by row as it is written
 int vk_launch_solve(const vk_plan *plan, const vk_columns *cols, const double *Dinv, const double *Lsub,
                     const double *Usup, const double *in, double *out, double *tmp, const vk_colstate *st,
-                    const vk_bcr_ws *ws, hipStream_t s) {
+                    const vk_bcr_ws *ws, hipStream_t s, double *acc = nullptr) {
     const int ni = plan->ni, nc = cols->ncols;
     const int *off = cols->d_off;
     if (plan->factor_variant == VK_FACTOR_BCR) {
         if (!ws || !ws->A) return VK_EINVAL;
-        return vk_launch_bcr_solve(ni, cols, Dinv, Lsub, Usup, in, out, tmp, st, ws, s);
+        return vk_launch_bcr_solve(ni, cols, Dinv, Lsub, Usup, in, out, tmp, st, ws, acc, s);
     }
     switch ((ni + 15) / 16) {
-    case 1: launch_solve_n<1>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 2: launch_solve_n<2>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 3: launch_solve_n<3>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 4: launch_solve_n<4>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 5: launch_solve_n<5>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 6: launch_solve_n<6>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 7: launch_solve_n<7>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 8: launch_solve_n<8>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 9: launch_solve_n<9>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
-    case 10: launch_solve_n<10>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, s); break;
+    case 1: launch_solve_n<1>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 2: launch_solve_n<2>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 3: launch_solve_n<3>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 4: launch_solve_n<4>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 5: launch_solve_n<5>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 6: launch_solve_n<6>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 7: launch_solve_n<7>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 8: launch_solve_n<8>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 9: launch_solve_n<9>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 10: launch_solve_n<10>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
     default: return VK_ETOOBIG;
     }
     VK_CHECK(hipGetLastError());
@@ -432,15 +431,12 @@ int vk_launch_solve_refined(const vk_plan *plan, const vk_columns *cols, const d
                             double *tmp, int refine, const vk_colstate *st, const vk_bcr_ws *ws, hipStream_t s) {
     int r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, b, x, tmp, st, ws, s);
     if (r) return r;
-    const int n = cols->L * plan->ni;
     for (int q = 0; q < refine; q++) {
         r = vk_launch_residual(plan, cols, D, Lsub, Usup, x, b, work, st, s);
         if (r) return r;
-        r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, work, work, tmp, st, ws, s);
+        // the correction is added into x by the solve's backward sweep
+        r = vk_launch_solve(plan, cols, Dinv, Lsub, Usup, work, work, tmp, st, ws, s, x);
         if (r) return r;
-        hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, n, plan->ni, cols->d_layer_col, work, x,
-                           st);
-        VK_CHECK(hipGetLastError());
     }
     return 0;
 }
