@@ -203,17 +203,18 @@ __device__ inline double dside(int4 t4, double kcol, const double *yl, double m,
     return tot;
 }
 
+// ysum[l] = sum_i y[l][i] in numpy's pairwise order, one thread per layer
+__global__ void k_ysum(int L, int ni, const double *__restrict__ y, double *__restrict__ ysum) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < L) ysum[l] = pw_sum(y + (size_t)l * ni, ni);
+}
 
 // the three layer totals a layer's diffusion terms use (0 outside the column)
-// the totals sum_i y[l'][i] of layers j-1, j, j+1 of the column (0 beyond
-// its ends) in numpy's pairwise order, by threads 0, 1, 2 into ys3 (read
-// after the caller's next barrier)
-__device__ inline void layer_sums(const double *y, int ni, const LayerCtx &x, int l, double *ys3) {
-    const int t = threadIdx.x;
-    if (t < 3) {
-        const int jj = x.j + t - 1;
-        ys3[t] = (jj >= 0 && jj < x.nz) ? pw_sum(y + (size_t)(l + t - 1) * ni, ni) : 0.0;
-    }
+__device__ inline void ysum_around(const double *ysum, const LayerCtx &x, double &ym, double &y0, double &yp) {
+    const double *ys = ysum + x.off;
+    ym = x.j > 0 ? ys[x.j - 1] : 0.0;
+    y0 = ys[x.j];
+    yp = x.j < x.nz - 1 ? ys[x.j + 1] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -227,9 +228,9 @@ __global__ void __launch_bounds__(BLK) k_rhs(
     const int2 *__restrict__ chem_ent, const double *__restrict__ ms,
     const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y, const double *__restrict__ k,
     const double *__restrict__ M, int mode, int mol, const double *__restrict__ k1,
-    const double *__restrict__ dtc, const vk_colstate *__restrict__ st, double *__restrict__ out) {
+    const double *__restrict__ dtc, const double *__restrict__ ysum, const vk_colstate *__restrict__ st,
+    double *__restrict__ out) {
     extern __shared__ double smem[];
-    __shared__ double ys3[3];
     double *yl = smem;              // ni
     double *v = smem + VK_NI_MAX;   // npairs
     const int l = blockIdx.x;
@@ -237,10 +238,9 @@ __global__ void __launch_bounds__(BLK) k_rhs(
     if (st && st[x.c].done) return;
     const double *yrow = y + (size_t)l * ni;
     for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
-    if (mode & 2) layer_sums(y, ni, x, l, ys3);
-    __syncthreads();
     double ysm = 0.0, ys0 = 0.0, ysp = 0.0;
-    if (mode & 2) { ysm = ys3[0]; ys0 = ys3[1]; ysp = ys3[2]; }
+    if (mode & 2) ysum_around(ysum, x, ysm, ys0, ysp);
+    __syncthreads();
     if (mode & 1) {
         const double *kl = k + (size_t)l * 2 * npairs;
         double m = M[l];
@@ -343,10 +343,9 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     const int *__restrict__ jac_tp,
     const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y,
     const double *__restrict__ k, const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
-    const vk_colstate *__restrict__ st, double *__restrict__ D, double *__restrict__ Lsub,
-    double *__restrict__ Usup) {
+    const double *__restrict__ ysum, const vk_colstate *__restrict__ st, double *__restrict__ D,
+    double *__restrict__ Lsub, double *__restrict__ Usup) {
     extern __shared__ double smem[];
-    __shared__ double ys3[3];
     double *yl = smem;                       // ni
     double *dg0 = smem + VK_NI_MAX;          // diffusion terms of the diagonal, per species
     double *dg1 = smem + 2 * VK_NI_MAX;
@@ -357,8 +356,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     if (st && st[x.c].done) return;
     const double *yrow = y + (size_t)l * ni;
     for (int i = threadIdx.x; i < ni; i += blockDim.x) yl[i] = yrow[i];
-    if (mode == 1) layer_sums(y, ni, x, l, ys3);
-    __syncthreads();
+    __syncthreads();   // the slot derivatives read yl
     const int nn = ni * ni;
     double *Dl = D + (size_t)l * nn;
     const double *kl = k + (size_t)l * 2 * npairs;
@@ -400,7 +398,8 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     if (mode == 1) {
         // diffusion part of the lhs (op.py:2012-2040): once per species
         c0 = 1. / (R_ROS * dtc[x.c]);
-        const double ysm = ys3[0], ys0 = ys3[1], ysp = ys3[2];
+        double ysm, ys0, ysp;
+        ysum_around(ysum, x, ysm, ys0, ysp);
         ColAtm ca = col_atm(atm, x.c, x.off, x.nz, ni);
         if (x.j == 0 && ca.use_bot) xmode = 2;
         else if (x.j == x.nz - 1 && ca.vm && mol && atm.top_flux && esc_any(atm)) xmode = 1;
@@ -493,23 +492,44 @@ extern "C" int vk_rates(const vk_plan *plan, const vk_columns *cols, const vk_at
     return 0;
 }
 
-int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
-                  const double *k, int mode, int mol, const double *k1, const double *dtc,
-                  const vk_colstate *st, double *out, hipStream_t s) {
-    vk_atm a = *atm;
-    hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, plan->nchem,
-                       cols->d_off,
-                       cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_ent, plan->ms,
-                       plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, st, out);
+static int launch_ysum(const vk_columns *cols, int ni, const double *y, double *ysum, hipStream_t s) {
+    hipLaunchKernelGGL(k_ysum, dim3((cols->L + 127) / 128), dim3(128), 0, s, cols->L, ni, y, ysum);
     VK_CHECK(hipGetLastError());
     return 0;
 }
 
+// ysum: scratch of L doubles (needed when the diffusion terms are evaluated)
+int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
+                  const double *k, int mode, int mol, const double *k1, const double *dtc,
+                  const vk_colstate *st, double *ysum, double *out, hipStream_t s) {
+    vk_atm a = *atm;
+    if (mode & 2) {
+        if (!ysum) return VK_EINVAL;
+        int r = launch_ysum(cols, plan->ni, y, ysum, s);
+        if (r) return r;
+    }
+    hipLaunchKernelGGL(k_rhs, dim3(cols->L), dim3(BLK), rhs_smem(plan), s, plan->ni, plan->npairs, plan->nchem,
+                       cols->d_off,
+                       cols->d_layer_col, plan->reac, plan->prod, plan->chem_ptr, plan->chem_ent, plan->ms,
+                       plan->alpha, a, y, k, atm->M, mode, mol, k1, dtc, ysum, st, out);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ysum_ready: the caller's earlier launch on s has left the layer sums of y in ysum
 int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
                       const double *k, const double *M, int mode, int mol, const double *dtc,
-                      const vk_colstate *st, double *D, double *Lsub, double *Usup, hipStream_t s) {
+                      const vk_colstate *st, double *ysum, double *D, double *Lsub, double *Usup, hipStream_t s,
+                      bool ysum_ready = false) {
     vk_atm a{};
     if (atm) a = *atm;
+    if (mode == 1) {
+        if (!ysum) return VK_EINVAL;
+        if (!ysum_ready) {
+            int r = launch_ysum(cols, plan->ni, y, ysum, s);
+            if (r) return r;
+        }
+    }
     // fewer layers than twice the CUs: split each layer's entries over up to 4 workgroups
     static const int ncu = [] {
         int dev = 0, n = 0;
@@ -522,36 +542,50 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L, gy), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        plan->njent, plan->njac, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
                        plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_tp, plan->ms, plan->alpha, a, y, k, M, mode, mol,
-                       dtc, st, D, Lsub, Usup);
+                       dtc, ysum, st, D, Lsub, Usup);
     VK_CHECK(hipGetLastError());
     return 0;
 }
+
+// temporary layer totals for the one-shot entry points
+struct YsumTmp {
+    double *p = nullptr;
+    hipStream_t s;
+    explicit YsumTmp(hipStream_t st) : s(st) {}
+    int alloc(int L) { return hipMallocAsync((void **)&p, sizeof(double) * (size_t)L, s) == hipSuccess ? 0 : VK_ENOMEM; }
+    ~YsumTmp() { if (p) (void)hipFreeAsync(p, s); }
+};
 
 extern "C" int vk_chemdf(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
                          const double *d_k, double *d_dydt, void *stream) {
     if (!plan || !cols || !d_y || !d_M || !d_k || !d_dydt) return VK_EINVAL;
     vk_atm a{};
     a.M = d_M;
-    return vk_launch_rhs(plan, cols, &a, d_y, d_k, 1, 0, nullptr, nullptr, nullptr, d_dydt, S(stream));
+    return vk_launch_rhs(plan, cols, &a, d_y, d_k, 1, 0, nullptr, nullptr, nullptr, nullptr, d_dydt, S(stream));
 }
 
 extern "C" int vk_diffdf(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
                          int use_moldiff, double *d_out, void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_out) return VK_EINVAL;
-    return vk_launch_rhs(plan, cols, atm, d_y, nullptr, 2, use_moldiff, nullptr, nullptr, nullptr, d_out, S(stream));
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_rhs(plan, cols, atm, d_y, nullptr, 2, use_moldiff, nullptr, nullptr, nullptr, ys.p, d_out,
+                         S(stream));
 }
 
 extern "C" int vk_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *d_y,
                       const double *d_k, int use_moldiff, const double *d_k1, const double *d_dt, double *d_out,
                       void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_k || !d_out || (d_k1 && !d_dt)) return VK_EINVAL;
-    return vk_launch_rhs(plan, cols, atm, d_y, d_k, 3, use_moldiff, d_k1, d_dt, nullptr, d_out, S(stream));
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_rhs(plan, cols, atm, d_y, d_k, 3, use_moldiff, d_k1, d_dt, nullptr, ys.p, d_out, S(stream));
 }
 
 extern "C" int vk_jac(const vk_plan *plan, const vk_columns *cols, const double *d_y, const double *d_M,
                       const double *d_k, double *d_J, void *stream) {
     if (!plan || !cols || !d_y || !d_M || !d_k || !d_J) return VK_EINVAL;
-    return vk_launch_jac_lhs(plan, cols, nullptr, d_y, d_k, d_M, 0, 0, nullptr, nullptr, d_J, nullptr, nullptr,
+    return vk_launch_jac_lhs(plan, cols, nullptr, d_y, d_k, d_M, 0, 0, nullptr, nullptr, nullptr, d_J, nullptr, nullptr,
                              S(stream));
 }
 
@@ -559,7 +593,9 @@ extern "C" int vk_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
                       const double *d_k, int use_moldiff, const double *d_dt, double *d_D, double *d_Lsub,
                       double *d_Usup, void *stream) {
     if (!plan || !cols || !atm || !d_y || !d_k || !d_dt || !d_D || !d_Lsub || !d_Usup) return VK_EINVAL;
-    return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, d_D, d_Lsub, d_Usup,
+    YsumTmp ys(S(stream));
+    if (ys.alloc(cols->L)) return VK_ENOMEM;
+    return vk_launch_jac_lhs(plan, cols, atm, d_y, d_k, atm->M, 1, use_moldiff, d_dt, nullptr, ys.p, d_D, d_Lsub, d_Usup,
                              S(stream));
 }
 

@@ -89,6 +89,7 @@ struct vk_integrator {
     double *b1, *x1, *b2, *x2, *work;   // right-hand sides and solutions of the two stages
     double *ztmp;                       // forward-sweep scratch of the twisted solve
     double *yk2, *part;
+    double *ysum;                       // layer totals for the diffusion terms
     vk_bcr_ws bcr;                      // block cyclic reduction workspace (allocated when selected)
     int *info;
     double *yhist;             // [history][L][ni]

@@ -12,10 +12,10 @@
 using namespace vk;
 
 int vk_launch_rhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *, int, int,
-                  const double *, const double *, const vk_colstate *, double *, hipStream_t);
+                  const double *, const double *, const vk_colstate *, double *, double *, hipStream_t);
 int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *,
                       const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
-                      hipStream_t);
+                      double *, hipStream_t, bool ysum_ready = false);
 int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
                      int *, const vk_colstate *, const vk_bcr_ws *, hipStream_t, bool info_zeroed = false);
 int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
@@ -699,6 +699,7 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
     r = r ? r : dalloc(v, &it->ztmp, L * ni);
     r = r ? r : dalloc(v, &it->yk2, L * ni);
     r = r ? r : dalloc(v, &it->part, L * NP);
+    r = r ? r : dalloc(v, &it->ysum, L);
     r = r ? r : dalloc(v, &it->info, nc);
     r = r ? r : dalloc(v, &it->flags, nc);
     r = r ? r : dalloc(v, &it->colscr, nc * 16);
@@ -799,9 +800,11 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     // dt per column lives in the state struct; gather it contiguously
     double *dt = it->colscr;
     hipLaunchKernelGGL(k_gather_dt, dim3((cl->ncols + 127) / 128), dim3(128), 0, s, cl->ncols, st, dt, it->info);
-    int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->b1, s);
+    int r = vk_launch_rhs(p, cl, atm, y, k, 3, prm->use_moldiff, nullptr, nullptr, st, it->ysum, it->b1, s);
     if (r) return r;
-    r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->D, it->Lsub, it->Usup, s);
+    // the right-hand side above has left the layer sums of y in it->ysum
+    r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->ysum, it->D, it->Lsub, it->Usup,
+                          s, true);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
     r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, &it->bcr, s, true);
@@ -812,7 +815,7 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     if (r) return r;
     const int n = (int)(L * ni);
     hipLaunchKernelGGL(k_yk2, dim3((n + 255) / 256), dim3(256), 0, s, n, ni, cl->d_layer_col, y, it->x1, st, it->yk2);
-    r = vk_launch_rhs(p, cl, atm, it->yk2, k, 3, prm->use_moldiff, it->x1, dt, st, it->b2,
+    r = vk_launch_rhs(p, cl, atm, it->yk2, k, 3, prm->use_moldiff, it->x1, dt, st, it->ysum, it->b2,
                       s);   // f(yk2) - 2/(r h) k1
     if (r) return r;
     r = vk_launch_solve_refined(p, cl, it->D, it->Dinv, it->Lsub, it->Usup, it->b2, it->x2, it->work, it->ztmp, prm->refine,
