@@ -393,8 +393,8 @@ int vk_photo_read(const vk_photo *photo, int what, double *h_out);
  * vk_integrate is given are rewritten in place.  NULL detaches. */
 int vk_integrator_set_photo(vk_integrator *it, vk_photo *photo, int ini_frq, int final_frq);
 
-/* ---- warm-start MLP (no counterpart in the reference: it seeds the plain
- * Ros2 integration, which stays the fallback and the judge) --------------
+/* ---- warm-start MLP (no counterpart in the reference: it only seeds the
+ * plain Ros2 integration, which still decides convergence) ---------------
  * BF16 tensors are passed as raw 16-bit arrays.  GEMM: C[m][n] = sum_k
  * A(m,k) B(k,n) (+ bias[n]) (+ C if accumulate) with A(m,k) at
  * A[m*sam + k*sak] and B(k,n) at B[k*sbk + n*sbn]; FP32 accumulation on
