@@ -16,7 +16,9 @@ The JSON line also carries
   on the launch stream (eager attempts right after the timed region; the
   timed region itself replays the captured per-attempt HIP graph) and the
   HBM traffic per factorisation measured with rocprofv3 byte counters
-  (read from the committed counter summary, TRAFFIC_FILE below);
+  (read from the committed counter summary, TRAFFIC_FILE below), and the
+  FP64 work that runs on the matrix cores by rocprofv3's MFMA counters
+  (``mfma_counted``, from MFMA_FILE);
 * ``cpu_baseline``: the CPU oracle (reference algorithm: dense lhs, band
   repack, two banded LAPACK solves per attempt) timed on the host in a
   single-threaded subprocess on a bounded sample, and (``aggregate``) in P
@@ -56,6 +58,7 @@ FP64_PEAK = 78.6e12        # FLOP/s (vector = matrix on MI355X)
 # per run, each scaled by its reading for a 512 MiB device copy); the file
 # gives the command that made it.
 TRAFFIC_FILE = os.path.join('profiles', 'hbm_traffic_ros2_attempt.md')
+MFMA_FILE = os.path.join('profiles', 'mfma_counters_factorisation.md')
 # the warm-start network (307 -> 4 x 1024 SiLU -> 93 x 150), trained on the
 # device on steady states the batched solver generated (python -m
 # vulcan_amd.warmstart_study generate / train); plain arrays, BF16 weights
@@ -76,6 +79,26 @@ def factor_traffic(section):
     for line in part[1].split('\n## ')[0].splitlines():
         if line.startswith('Factorisation traffic'):
             return float(line.split()[2]) * 1e6
+    return None
+
+
+def factor_mfma(section):
+    """Counted FP64 MFMA work of the factorisation in a workload section of
+    the MFMA counter summary: {'gflop', 'tflops', 'busy_frac'} per attempt
+    (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 and SQ_VALU_MFMA_BUSY_CYCLES over the
+    dispatch time), or None."""
+    try:
+        text = open(os.path.join(ROOT, MFMA_FILE)).read()
+    except OSError:
+        return None
+    part = text.split('## ' + section, 1)
+    if len(part) < 2:
+        return None
+    for line in part[1].split('\n## ')[0].splitlines():
+        if line.startswith('| factorisation |'):
+            f = [c.strip() for c in line.strip('|').split('|')]
+            return {'gflop': float(f[1]), 'tflops': float(f[5]), 'busy_frac': float(f[6]) / 100,
+                    'source': MFMA_FILE}
     return None
 
 
@@ -420,7 +443,8 @@ def main():
                      'traffic_unit': 'bytes per factorisation (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)',
                      'traffic_source': TRAFFIC_FILE,
                      'algorithmic_bytes_per_factorisation': nz * 2 * ni * ni * 8,
-                     'flops_per_factorisation': flops_launch, 'avg_factorisation_ms': s['factor_ms']},
+                     'flops_per_factorisation': flops_launch, 'avg_factorisation_ms': s['factor_ms'],
+                     'mfma_counted': factor_mfma('one column')},
         'kernels': {'workload': 'the same column', 'unit': 'GB/s (algorithmic bytes / HIP-event time)',
                     'hbm_peak_GBps': HBM_PEAK / 1e9, **s['kernels']},
     }
@@ -435,6 +459,7 @@ def main():
                           'factor_tflops': fl / (bt['factor_ms'] * 1e-3) / 1e12,
                           'factor_frac': fl / (bt['factor_ms'] * 1e-3) / FP64_PEAK,
                           'factor_traffic': factor_traffic('256 columns'),
+                          'factor_mfma_counted': factor_mfma('256 columns'),
                           'vs_single': (bt['col_steps'] / bt['time']) / value,
                           'kernels': bt['kernels']}
     if not args.no_mixed:
