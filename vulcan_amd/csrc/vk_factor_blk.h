@@ -9,12 +9,15 @@
 namespace {
 
 // 8-wave workgroups (blocks above 112 rows) run one per CU
+// (order: the chain each workgroup runs, 2c + side; null = blockIdx.x)
 template <int NB, int MINW>
-__global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : MINW) k_factor_blk(int ni, const int *__restrict__ off, const double *__restrict__ D,
+__global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : MINW) k_factor_blk(int ni, const int *__restrict__ off,
+                                                    const int *__restrict__ order, const double *__restrict__ D,
                                                     const double *__restrict__ Lsub, const double *__restrict__ Usup,
                                                     double *__restrict__ Dinv, int *__restrict__ info,
                                                     const vk_colstate *__restrict__ st) {
-    const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int chain = order ? order[blockIdx.x] : (int)blockIdx.x;
+    const int c = chain >> 1, side = chain & 1;
     if (st && st[c].done) return;
     __shared__ BlkShared<NB> sh;
     if (threadIdx.x == 0) sh.info = 0;
@@ -58,28 +61,28 @@ __global__ void __launch_bounds__(blk_nt(NB), blk_nt(NB) > NT ? 1 : MINW) k_fact
 }
 
 template <int N, int MINW>
-void launch_blk(int ni, int ncols, const int *off, const double *D, const double *Lsub, const double *Usup,
-                double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_factor_blk<N, MINW>), dim3(2 * ncols), dim3(blk_nt(N)), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
-                       st);
+void launch_blk(int ni, int ncols, const int *off, const int *order, const double *D, const double *Lsub,
+                const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
+    hipLaunchKernelGGL((k_factor_blk<N, MINW>), dim3(2 * ncols), dim3(blk_nt(N)), 0, s, ni, off, order, D, Lsub, Usup,
+                       Dinv, info, st);
     hipLaunchKernelGGL((k_factor_blk_mid<N, MINW>), dim3(ncols), dim3(blk_nt(N)), 0, s, ni, off, D, Lsub, Usup, Dinv, info,
                        st);
 }
 
 template <int MINW>
-int launch_blocked(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
+int launch_blocked(int nb, int ni, int ncols, const int *off, const int *order, const double *D, const double *Lsub,
                    const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
     switch (nb) {
-    case 1: launch_blk<1, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 2: launch_blk<2, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 3: launch_blk<3, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 4: launch_blk<4, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 5: launch_blk<5, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 6: launch_blk<6, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 7: launch_blk<7, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 8: launch_blk<8, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 9: launch_blk<9, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
-    case 10: launch_blk<10, MINW>(ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 1: launch_blk<1, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 2: launch_blk<2, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 3: launch_blk<3, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 4: launch_blk<4, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 5: launch_blk<5, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 6: launch_blk<6, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 7: launch_blk<7, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 8: launch_blk<8, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 9: launch_blk<9, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
+    case 10: launch_blk<10, MINW>(ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s); break;
     default: return VK_ETOOBIG;
     }
     return 0;

@@ -2,7 +2,8 @@
 // (factor variant 2).
 #include "vk_factor_blk.h"
 
-int vk_launch_factor_blocked2(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
-                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s) {
-    return launch_blocked<2>(nb, ni, ncols, off, D, Lsub, Usup, Dinv, info, st, s);
+int vk_launch_factor_blocked2(int nb, int ni, int ncols, const int *off, const int *order, const double *D,
+                              const double *Lsub, const double *Usup, double *Dinv, int *info, const vk_colstate *st,
+                              hipStream_t s) {
+    return launch_blocked<2>(nb, ni, ncols, off, order, D, Lsub, Usup, Dinv, info, st, s);
 }

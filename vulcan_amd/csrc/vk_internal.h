@@ -69,6 +69,14 @@ struct vk_columns {
     std::vector<int> off;      // host copy, ncols+1
     int *d_off;                // ncols+1
     int *d_layer_col;          // L
+    // chain order of the blocked chain factorisations: workgroup b runs
+    // chain order[b] (2c: top half of column c, 2c+1: bottom half).
+    // [0]: one workgroup per CU, longest chains first (a freed CU takes the
+    // longest chain left); [1]: two per CU, where workgroups b and b + nCU
+    // share a CU (first nCU workgroups one per CU), pairing the longest
+    // chains with the shortest.  The order moves chains between CUs only;
+    // each chain's arithmetic is the same wherever it runs.
+    int *d_chain_order[2];     // 2 * ncols each
     vk_bcr_sched bcr;
 };
 
@@ -113,10 +121,12 @@ struct vk_integrator {
 // one translation unit each); 0 or a VK_E* code.
 int vk_launch_factor_register(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
                               const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
-int vk_launch_factor_blocked1(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
-                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
-int vk_launch_factor_blocked2(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
-                              const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
+int vk_launch_factor_blocked1(int nb, int ni, int ncols, const int *off, const int *order, const double *D,
+                              const double *Lsub, const double *Usup, double *Dinv, int *info, const vk_colstate *st,
+                              hipStream_t s);
+int vk_launch_factor_blocked2(int nb, int ni, int ncols, const int *off, const int *order, const double *D,
+                              const double *Lsub, const double *Usup, double *Dinv, int *info, const vk_colstate *st,
+                              hipStream_t s);
 int vk_launch_factor_wave(int nb, int ni, int ncols, const int *off, const double *D, const double *Lsub,
                           const double *Usup, double *Dinv, int *info, const vk_colstate *st, hipStream_t s);
 int vk_launch_bcr_factor(int nb, int ni, const vk_columns *cols, const double *D, const double *Lsub,

@@ -627,6 +627,49 @@ static int bcr_schedule(vk_columns *c) {
     return r;
 }
 
+// Chain orders of the blocked chain factorisations (vk_columns::d_chain_order).
+// Chain 2c runs the top m = nz/2 layers of column c, chain 2c+1 the bottom
+// nz-1-m; the ranks sort them longest first (ties by index).  ncu: compute
+// units of the device.
+static int chain_orders(const std::vector<int> &off, int ncu, std::vector<int> &lpt, std::vector<int> &pair) {
+    const int ncols = (int)off.size() - 1, n = 2 * ncols;
+    std::vector<int> len(n);
+    for (int c = 0; c < ncols; c++) {
+        const int nz = off[c + 1] - off[c], m = nz / 2;
+        len[2 * c] = m;
+        len[2 * c + 1] = nz - 1 - m;
+    }
+    lpt.resize(n);
+    for (int i = 0; i < n; i++) lpt[i] = i;
+    std::stable_sort(lpt.begin(), lpt.end(), [&](int a, int b) { return len[a] > len[b]; });
+    pair = lpt;
+    const int H = ncu;
+    if (H > 0 && n > H && n <= 2 * H) {
+        // the longest H-k chains alone on their CUs, the next k paired with
+        // the k shortest, longest with shortest
+        const int k = n - H;
+        for (int j = 0; j < k; j++) pair[j] = lpt[H - k + j];
+        for (int j = k; j < H; j++) pair[j] = lpt[j - k];
+        for (int j = 0; j < k; j++) pair[H + j] = lpt[n - 1 - j];
+    } else if (H > 0 && n > 2 * H) {
+        // the longest 2H chains paired among themselves; the rest follow, longest first
+        for (int j = 0; j < H; j++) {
+            pair[j] = lpt[j];
+            pair[H + j] = lpt[2 * H - 1 - j];
+        }
+    }
+    // both must be permutations of the chains
+    for (const std::vector<int> *o : {&lpt, &pair}) {
+        std::vector<char> seen(n, 0);
+        for (int b = 0; b < n; b++) {
+            const int v = (*o)[b];
+            if (v < 0 || v >= n || seen[v]) return VK_EINVAL;
+            seen[v] = 1;
+        }
+    }
+    return 0;
+}
+
 extern "C" int vk_columns_create(int ncols, const int *nz, vk_columns **out) {
     if (ncols <= 0 || !nz || !out) return VK_EINVAL;
     vk_columns *c = new vk_columns();
@@ -649,7 +692,20 @@ extern "C" int vk_columns_create(int ncols, const int *nz, vk_columns **out) {
     }
     VK_CHECK(hipMemcpy(c->d_off, c->off.data(), sizeof(int) * (ncols + 1), hipMemcpyHostToDevice));
     VK_CHECK(hipMemcpy(c->d_layer_col, lc.data(), sizeof(int) * c->L, hipMemcpyHostToDevice));
-    int r = bcr_schedule(c);
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess)
+        ncu = 0;
+    std::vector<int> ord[2];
+    int r = chain_orders(c->off, ncu, ord[0], ord[1]);
+    for (int v = 0; v < 2 && !r; v++) {
+        if (hipMalloc((void **)&c->d_chain_order[v], sizeof(int) * 2 * ncols) != hipSuccess) r = VK_ENOMEM;
+        else if (hipMemcpy(c->d_chain_order[v], ord[v].data(), sizeof(int) * 2 * ncols, hipMemcpyHostToDevice) !=
+                 hipSuccess)
+            r = VK_ENOMEM;
+    }
+    if (r) { vk_columns_destroy(c); return r; }
+    r = bcr_schedule(c);
     if (r) { vk_columns_destroy(c); return r; }
     *out = c;
     return 0;
@@ -659,6 +715,8 @@ extern "C" int vk_columns_destroy(vk_columns *c) {
     if (!c) return 0;
     (void)hipFree(c->d_off);
     (void)hipFree(c->d_layer_col);
+    for (int *o : c->d_chain_order)
+        if (o) (void)hipFree(o);
     if (c->bcr.d_elim) (void)hipFree(c->bcr.d_elim);
     if (c->bcr.d_surv) (void)hipFree(c->bcr.d_surv);
     if (c->bcr.d_sflag) (void)hipFree(c->bcr.d_sflag);

@@ -357,10 +357,10 @@ int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *
         if (!ws || !ws->A) return VK_EINVAL;
         return vk_launch_bcr_factor(n, ni, cols, D, Lsub, Usup, Dinv, info, st, ws, s);
     case VK_FACTOR_BLOCKED:
-        r = vk_launch_factor_blocked1(n, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s);
+        r = vk_launch_factor_blocked1(n, ni, nc, cols->d_off, cols->d_chain_order[0], D, Lsub, Usup, Dinv, info, st, s);
         break;
     case VK_FACTOR_BLOCKED_OCC2:
-        r = vk_launch_factor_blocked2(n, ni, nc, cols->d_off, D, Lsub, Usup, Dinv, info, st, s);
+        r = vk_launch_factor_blocked2(n, ni, nc, cols->d_off, cols->d_chain_order[1], D, Lsub, Usup, Dinv, info, st, s);
         break;
     case VK_FACTOR_WAVE:
         if (n <= 7) {
