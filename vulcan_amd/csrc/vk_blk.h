@@ -294,6 +294,17 @@ __device__ __forceinline__ void blk_invert(f64x4 (&acc)[BlkGrid<NB>::TBR][BlkGri
 }
 
 // acc = D_l - diag(cl) Xn^T diag(cn) [- diag(cl2) Xn2^T diag(cn2)] in the tile layout
+//
+// Branch-free: every thread loads all its entries unconditionally (padding
+// rows and columns read element 0 and clear it with a bit mask, giving the
+// +0.0 they hold), so a row block's loads are all in flight before the first
+// wait.  With a conditional load per entry (or a select on the loaded
+// value, which the compiler turns back into a branch) it waited on each
+// entry's loads in turn.  Xn and Xn2 are uniform.
+__device__ __forceinline__ double blk_masked(double v, unsigned long long m) {
+    return __longlong_as_double(__double_as_longlong(v) & (long long)m);
+}
+
 template <int NB>
 __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[BlkGrid<NB>::TBR][BlkGrid<NB>::TBC], int ni,
                                                const double *__restrict__ Dl, const double *cl, const double *Xn,
@@ -301,32 +312,76 @@ __device__ __forceinline__ void blk_load_schur(f64x4 (&acc)[BlkGrid<NB>::TBR][Bl
                                                const double *cn2) {
     using G = BlkGrid<NB>;
     constexpr int TBR = G::TBR, TBC = G::TBC, WC = G::WC;
-    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    int tid = threadIdx.x;
+    // opaque to the optimiser: the indices below are formed here, at each
+    // call, rather than hoisted out of the caller's loop and kept live in
+    // registers through the inversion
+    asm volatile("" : "+v"(tid));
+    const int w = tid >> 6, l = tid & 63;
     const int wr = w / WC, wc = w % WC;
     const int lg = l >> 4, lc = l & 15;
+    // column cc = 16 bj + lc of each column tile (index 0 and mask 0 outside the block)
+    int cj[TBC];
+    unsigned long long cm[TBC];
+#pragma unroll
+    for (int tj = 0; tj < TBC; tj++) {
+        const int bj = wc + WC * tj, cc = 16 * bj + lc;
+        const bool ok = bj < NB && cc < ni;
+        cj[tj] = ok ? cc : 0;
+        cm[tj] = ok ? ~0ull : 0ull;
+    }
+    // row i = 16 bi + 4 r + lg of each entry (index 0 and mask 0 outside the block)
+    int ir[TBR][4];
+    unsigned long long rm[TBR][4];
 #pragma unroll
     for (int ti = 0; ti < TBR; ti++) {
         const int bi = wr + 2 * ti;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int i = 16 * bi + 4 * r + lg;
-            const bool rowok = bi < NB && i < ni;
-            const double li = (Xn && rowok) ? cl[i] : 0.0;
-            const double li2 = (Xn2 && rowok) ? cl2[i] : 0.0;
-#pragma unroll
-            for (int tj = 0; tj < TBC; tj++) {
-                const int bj = wc + WC * tj;
-                const int cc = 16 * bj + lc;
-                double v = 0.0;
-                if (rowok && bj < NB && cc < ni) {
-                    v = Dl[(size_t)i * ni + cc];
-                    if (Xn) v = v - li * (Xn[(size_t)cc * ni + i] * cn[cc]);
-                    if (Xn2) v = v - li2 * (Xn2[(size_t)cc * ni + i] * cn2[cc]);
-                }
-                acc[ti][tj][r] = v;
-            }
+            const bool rok = bi < NB && i < ni;
+            ir[ti][r] = rok ? i : 0;
+            rm[ti][r] = rok ? ~0ull : 0ull;
         }
     }
+    // all of D's loads first, then (uniform) the Schur terms' loads
+#pragma unroll
+    for (int ti = 0; ti < TBR; ti++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int tj = 0; tj < TBC; tj++)
+                acc[ti][tj][r] = blk_masked(Dl[(size_t)ir[ti][r] * ni + cj[tj]], rm[ti][r] & cm[tj]);
+    // the Schur terms in groups of GR row tiles (all at once up to 96 rows;
+    // one tile at a time above, where their loads would not fit the registers)
+    constexpr int GR = TBR * TBC <= 9 ? TBR : 1;
+    auto sub = [&](const double *c_l, const double *X, const double *c_n) {
+        double cnv[TBC];
+#pragma unroll
+        for (int tj = 0; tj < TBC; tj++) cnv[tj] = c_n[cj[tj]];
+#pragma unroll
+        for (int t0 = 0; t0 < TBR; t0 += GR) {
+            double li[GR][4], x[GR][4][TBC];
+#pragma unroll
+            for (int g = 0; g < GR; g++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    li[g][r] = c_l[ir[t0 + g][r]];
+#pragma unroll
+                    for (int tj = 0; tj < TBC; tj++) x[g][r][tj] = X[(size_t)cj[tj] * ni + ir[t0 + g][r]];
+                }
+#pragma unroll
+            for (int g = 0; g < GR; g++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int tj = 0; tj < TBC; tj++)
+                        acc[t0 + g][tj][r] = blk_masked(acc[t0 + g][tj][r] - li[g][r] * (x[g][r][tj] * cnv[tj]),
+                                                        rm[t0 + g][r] & cm[tj]);
+        }
+    };
+    if (Xn) sub(cl, Xn, cn);
+    if (Xn2) sub(cl2, Xn2, cn2);
 }
 
 // Products for the cyclic-reduction updates, one 32-row output slab (and
