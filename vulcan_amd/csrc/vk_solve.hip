@@ -282,11 +282,13 @@ __device__ inline void dd_sub_prod(double &hi, double &lo, double a, double b) {
     lo = lo + (se - pe);
 }
 
-// r = b - A x, 32 rows of one layer per workgroup: the rows of D are staged
-// in LDS with coalesced loads, 8 threads share a row (each a double-double
-// partial over an eighth of the columns), and the partials, b and the two
-// coupling terms are combined in double-double.
+// r = b - A x, 32 rows of one layer per workgroup: 8 threads share a row,
+// each a double-double partial over an eighth of the columns (a contiguous
+// run, read straight into registers: every load of the run is issued
+// before the first product), and the partials, b and the two coupling
+// terms are combined in double-double.
 constexpr int RES_ROWS = 32, RES_PARTS = NT / RES_ROWS;
+constexpr int RES_CH = (VK_NI_MAX + RES_PARTS - 1) / RES_PARTS;   // longest run
 
 __device__ inline void dd_add(double &hi, double &lo, double a_hi, double a_lo) {
     const double s = hi + a_hi;
@@ -302,8 +304,7 @@ __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__
                                                   const double *__restrict__ x, const double *__restrict__ b,
                                                   double *__restrict__ r, const vk_colstate *__restrict__ st) {
     extern __shared__ double rsm[];
-    double *Ds = rsm;                        // RES_ROWS x ni
-    double *xs = rsm + RES_ROWS * ni;        // ni
+    double *xs = rsm;                        // ni
     double *ph = xs + ni;                    // RES_ROWS x RES_PARTS partial sums (hi)
     double *pl = ph + NT;                    // (lo)
     const int l = blockIdx.x;
@@ -314,15 +315,22 @@ __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__
     const int nrows = min(RES_ROWS, ni - i0);
     const double *Dl = D + ((size_t)l * ni + i0) * ni;
     const int tid = threadIdx.x;
-    for (int e = tid; e < nrows * ni; e += NT) Ds[e] = Dl[e];
-    for (int m = tid; m < ni; m += NT) xs[m] = x[(size_t)l * ni + m];
-    __syncthreads();
     const int rr = tid / RES_PARTS, pp = tid % RES_PARTS;
     const int chunk = (ni + RES_PARTS - 1) / RES_PARTS;
+    const int m0 = pp * chunk, m1 = min(ni, m0 + chunk);
+    // this thread's run of its row (indices clamped into the row; entries
+    // past the run are loaded but not used)
+    double dr[RES_CH];
+    const double *row = Dl + (size_t)min(rr, nrows - 1) * ni;
+#pragma unroll
+    for (int k = 0; k < RES_CH; k++) dr[k] = row[min(m0 + k, ni - 1)];
+    for (int m = tid; m < ni; m += NT) xs[m] = x[(size_t)l * ni + m];
+    __syncthreads();
     double hi = 0.0, lo = 0.0;
     if (rr < nrows) {
-        const int m1 = min(ni, (pp + 1) * chunk);
-        for (int m = pp * chunk; m < m1; m++) dd_sub_prod(hi, lo, Ds[rr * ni + m], xs[m]);
+#pragma unroll
+        for (int k = 0; k < RES_CH; k++)
+            if (m0 + k < m1) dd_sub_prod(hi, lo, dr[k], xs[m0 + k]);
     }
     ph[tid] = hi;
     pl[tid] = lo;
@@ -418,7 +426,7 @@ int vk_launch_residual(const vk_plan *plan, const vk_columns *cols, const double
                        const double *Usup, const double *x, const double *b, double *r, const vk_colstate *st,
                        hipStream_t s) {
     const int ni = plan->ni;
-    const size_t smem = sizeof(double) * ((size_t)RES_ROWS * ni + ni + 2 * NT);
+    const size_t smem = sizeof(double) * ((size_t)ni + 2 * NT);
     hipLaunchKernelGGL(k_residual, dim3(cols->L, (ni + RES_ROWS - 1) / RES_ROWS), dim3(NT), smem, s, ni, cols->d_off,
                        cols->d_layer_col, D, Lsub, Usup, x, b, r, st);
     VK_CHECK(hipGetLastError());
