@@ -87,6 +87,13 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
 // largest magnitude to 12 mantissa bits, ties to the lowest row.  The pivot
 // bookkeeping (ipos of the chosen rows, perm of the steps) stays in
 // registers until the panel ends: no LDS write waits inside a step.
+//
+// A step subtracts (v[kk] / pivot) x the pivot row from every other row.
+// The update is linear in the row, so a chosen pivot row is kept unscaled
+// (its own multiplier is 0 and its pivot-column entry 1): later steps update
+// it like any row, and the panel's end multiplies it by its 1 / pivot,
+// giving the Gauss-Jordan values (the scaled row, 1 / pivot in the pivot
+// column) without scaling the row inside the step.
 template <int NB>
 __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * NB + 1], int ni, int p0, int kend,
                                           int tag) {
@@ -97,6 +104,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
     // this lane's row q: -1 still available, -2 not a candidate (padding or
     // chosen in an earlier panel), >= 0 the pivot step that chose it
     int code[RPL];
+    double pinv[RPL];   // 1 / pivot of a row chosen in this panel
     bool zero = false;
 #pragma unroll
     for (int q = 0; q < RPL; q++) {
@@ -104,6 +112,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
 #pragma unroll
         for (int k = 0; k < 16; k++) v[q][k] = (r < N16) ? pan[k][r] : 0.0;
         code[q] = (r < ni) && sh.ipos[r] < 0 ? -1 : -2;
+        pinv[q] = 1.0;
     }
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
@@ -135,23 +144,15 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         for (int k = 0; k < 16; k++) rs[k] = sh.prow[k];
         const double inv = 1.0 / rs[kk];
 #pragma unroll
-        for (int k = 0; k < 16; k++) rs[k] = rs[k] * inv;
-#pragma unroll
         for (int q = 0; q < RPL; q++) {
-            const double f = v[q][kk];
+            const bool own = q == qp && l == pl;
+            const double f = own ? 0.0 : v[q][kk] * inv;
 #pragma unroll
             for (int k = 0; k < 16; k++) v[q][k] = fma(-f, rs[k], v[q][k]);
-            v[q][kk] = -f * inv;
+            v[q][kk] = own ? 1.0 : -f;
+            code[q] = own ? p0 + kk : code[q];
+            pinv[q] = own ? inv : pinv[q];
         }
-        // the pivot row itself: the scaled row, inv in the pivot column
-#pragma unroll
-        for (int q = 0; q < RPL; q++)
-            if (q == qp && l == pl) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) v[q][k] = rs[k];
-                v[q][kk] = inv;
-                code[q] = p0 + kk;
-            }
         if (l == 0) sh.perm[p0 + kk] = p;
     }
     if (zero && l == 0 && sh.info == 0) sh.info = tag;
@@ -160,8 +161,9 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         const int r = l + 64 * q;
         if (code[q] >= 0) sh.ipos[r] = code[q];
         if (r < N16) {
+            // rows chosen in this panel take their 1 / pivot (the others 1.0, exact)
 #pragma unroll
-            for (int k = 0; k < 16; k++) pan[k][r] = v[q][k];
+            for (int k = 0; k < 16; k++) pan[k][r] = v[q][k] * pinv[q];
         }
     }
     for (int s = kend; s < 16; s++)
