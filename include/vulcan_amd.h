@@ -250,6 +250,10 @@ int vk_selftest_mfma_f64(double *d_out, void *stream);
  * search (32-bit keys): out[l] = max over the 64 lanes of in[], for every
  * lane l. */
 int vk_selftest_wave_max_u32(const unsigned *d_in, unsigned *d_out, void *stream);
+/* Self-test of the reciprocal of the blocked inversion's pivot step (the
+ * hardware estimate refined by two Newton-Raphson steps): d_out[i] =
+ * that reciprocal of d_in[i], for i < n. */
+int vk_selftest_recip_f64(int n, const double *d_in, double *d_out, void *stream);
 
 /* replaces the gbtrs half of solve_banded (op.py:2914, 2929): rhs <- lhs^-1 rhs */
 int vk_block_solve(const vk_plan *plan, const vk_columns *cols, const double *d_Dinv,

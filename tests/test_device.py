@@ -202,6 +202,28 @@ def test_wave_max_u32():
         assert (out.cpu().numpy().view(np.uint32) == v.max()).all(), trial
 
 
+@gpu
+def test_pivot_reciprocal_matches_ieee_division():
+    """The pivot step's reciprocal (hardware estimate and two Newton-Raphson
+    steps) against IEEE 1/d computed on the host, over magnitudes 2^-600 to
+    2^600 of both signs and the powers of two; it is allowed one ulp, and
+    in practice is exact."""
+    torch = _torch()
+    from vulcan_amd.device import dptr, stream_handle
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    d = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-600, 601, n)) * rng.choice([-1.0, 1.0], n)
+    d[:1201] = np.ldexp(1.0, np.arange(-600, 601))
+    vin = torch.as_tensor(d, device='cuda')
+    out = torch.empty_like(vin)
+    _lib.check(_lib.load().vk_selftest_recip_f64(n, dptr(vin), dptr(out), stream_handle()), 'vk_selftest_recip_f64')
+    got = out.cpu().numpy()
+    ref = 1.0 / d
+    ulp = np.abs(got.view(np.int64) - ref.view(np.int64))
+    print('max ulp %d, %d of %d differ' % (ulp.max(), int((ulp != 0).sum()), n))
+    assert ulp.max() <= 1
+
+
 FACTOR_VARIANTS = [0, 1, 2, 3, 4]   # register GJ, blocked MFMA (1 or 2 WGs per CU), two-wave, cyclic reduction
 
 

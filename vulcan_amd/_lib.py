@@ -82,6 +82,7 @@ _SIGS = {
     'vk_block_solve_system_noalloc': [VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP],
     'vk_selftest_mfma_f64': [VP, VP],
     'vk_selftest_wave_max_u32': [VP, VP, VP],
+    'vk_selftest_recip_f64': [C.c_int, VP, VP, VP],
     'vk_block_solve': [VP, VP, VP, VP, VP, VP, VP],
     'vk_block_residual': [VP, VP, VP, VP, VP, VP, VP, VP, VP],
     'vk_integrator_create': [VP, VP, C.c_int, C.POINTER(VP)],

@@ -82,6 +82,18 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// 1 / d: the hardware reciprocal estimate refined by two Newton-Raphson
+// steps (e = 1 - d r, r = r + r e), within about an ulp; the IEEE division
+// sequence (scale, estimate, refinement, fix-up) is twice as long on the
+// pivot step's path.
+__device__ __forceinline__ double blk_recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Pivot steps p0 .. p0+kend-1 on the panel pan[k][row] (one wave).  The
 // pivot key is the upper word of |value| with the row in its low byte: the
 // largest magnitude to 12 mantissa bits, ties to the lowest row.  The pivot
@@ -142,7 +154,7 @@ __device__ __forceinline__ void blk_panel(BlkShared<NB> &sh, double (*pan)[16 * 
         double rs[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) rs[k] = sh.prow[k];
-        const double inv = 1.0 / rs[kk];
+        const double inv = blk_recip(rs[kk]);
 #pragma unroll
         for (int q = 0; q < RPL; q++) {
             const bool own = q == qp && l == pl;

@@ -676,7 +676,19 @@ namespace {
 __global__ void k_selftest_wave_max(const unsigned *in, unsigned *out) {
     out[threadIdx.x] = wave_max_u32(in[threadIdx.x]);
 }
+__global__ void k_selftest_recip(int n, const double *in, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = blk_recip(in[i]);
+}
 }  // namespace
+
+extern "C" int vk_selftest_recip_f64(int n, const double *d_in, double *d_out, void *stream) {
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return VK_EINVAL;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_selftest_recip, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, d_in, d_out);
+    VK_CHECK(hipGetLastError());
+    return 0;
+}
 
 extern "C" int vk_selftest_wave_max_u32(const unsigned *d_in, unsigned *d_out, void *stream) {
     if (!d_in || !d_out) return VK_EINVAL;
