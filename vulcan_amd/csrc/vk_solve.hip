@@ -140,12 +140,13 @@ __device__ __forceinline__ double solve_matvec(const double (&cur)[CH], const do
 }
 
 template <int NB>
-__global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restrict__ off,
+__global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restrict__ off, const int *__restrict__ order,
                                                      const double *__restrict__ Dinv, const double *__restrict__ Lsub,
                                                      const double *__restrict__ Usup, const double *in, double *tmp,
                                                      const vk_colstate *__restrict__ st) {
     constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
-    const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int chain = order ? order[blockIdx.x] : (int)blockIdx.x;   // 2c + side
+    const int c = chain >> 1, side = chain & 1;
     if (st && st[c].done) return;
     const int tid = threadIdx.x, i = tid % RP, h = tid / RP;
     const ChunkMap cm = chunk_map<SP, CH>(ni, i, h);
@@ -202,12 +203,13 @@ __global__ void __launch_bounds__(NTS) k_solve_fwd2(int ni, const int *__restric
 }
 
 template <int NB>
-__global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restrict__ off,
+__global__ void __launch_bounds__(NTS) k_solve_bwd2(int ni, const int *__restrict__ off, const int *__restrict__ order,
                                                      const double *__restrict__ Dinv, const double *__restrict__ Lsub,
                                                      const double *__restrict__ Usup, const double *tmp, double *out,
                                                      const vk_colstate *__restrict__ st, double *acc) {
     constexpr int RP = SolveShape<NB>::RP, SP = SolveShape<NB>::SP, CH = SolveShape<NB>::CH;
-    const int c = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int chain = order ? order[blockIdx.x] : (int)blockIdx.x;   // 2c + side
+    const int c = chain >> 1, side = chain & 1;
     if (st && st[c].done) return;
     const int tid = threadIdx.x, i = tid % RP, h = tid / RP;
     const ChunkMap cm = chunk_map<SP, CH>(ni, i, h);
@@ -386,12 +388,16 @@ int vk_launch_factor(const vk_plan *plan, const vk_columns *cols, const double *
 
 // out = lhs^-1 in (tmp: L*ni scratch, may not alias in/out; in may alias out)
 namespace {
+// order: the chain each workgroup runs (the pairing of the two-per-CU
+// factorisation: long chains share a CU with short ones)
 template <int NB>
-void launch_solve_n(int ni, int ncols, const int *off, const double *Dinv, const double *Lsub, const double *Usup,
-                    const double *in, double *out, double *tmp, const vk_colstate *st, double *acc, hipStream_t s) {
-    hipLaunchKernelGGL((k_solve_fwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, in, tmp, st);
-    hipLaunchKernelGGL((k_solve_bwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, Dinv, Lsub, Usup, tmp, out, st,
-                       acc);
+void launch_solve_n(int ni, int ncols, const int *off, const int *order, const double *Dinv, const double *Lsub,
+                    const double *Usup, const double *in, double *out, double *tmp, const vk_colstate *st, double *acc,
+                    hipStream_t s) {
+    hipLaunchKernelGGL((k_solve_fwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, order, Dinv, Lsub, Usup, in, tmp,
+                       st);
+    hipLaunchKernelGGL((k_solve_bwd2<NB>), dim3(2 * ncols), dim3(NTS), 0, s, ni, off, order, Dinv, Lsub, Usup, tmp, out,
+                       st, acc);
 }
 }  // namespace
 
@@ -406,16 +412,16 @@ int vk_launch_solve(const vk_plan *plan, const vk_columns *cols, const double *D
         return vk_launch_bcr_solve(ni, cols, Dinv, Lsub, Usup, in, out, tmp, st, ws, acc, s);
     }
     switch ((ni + 15) / 16) {
-    case 1: launch_solve_n<1>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 2: launch_solve_n<2>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 3: launch_solve_n<3>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 4: launch_solve_n<4>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 5: launch_solve_n<5>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 6: launch_solve_n<6>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 7: launch_solve_n<7>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 8: launch_solve_n<8>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 9: launch_solve_n<9>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
-    case 10: launch_solve_n<10>(ni, nc, off, Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 1: launch_solve_n<1>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 2: launch_solve_n<2>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 3: launch_solve_n<3>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 4: launch_solve_n<4>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 5: launch_solve_n<5>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 6: launch_solve_n<6>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 7: launch_solve_n<7>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 8: launch_solve_n<8>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 9: launch_solve_n<9>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
+    case 10: launch_solve_n<10>(ni, nc, off, cols->d_chain_order[1], Dinv, Lsub, Usup, in, out, tmp, st, acc, s); break;
     default: return VK_ETOOBIG;
     }
     VK_CHECK(hipGetLastError());
