@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     const double *__restrict__ ms, const double *__restrict__ alpha, vk_atm atm, const double *__restrict__ y,
     const double *__restrict__ k, const double *__restrict__ M, int mode, int mol, const double *__restrict__ dtc,
     const double *__restrict__ ysum, const vk_colstate *__restrict__ st, double *__restrict__ D,
-    double *__restrict__ Lsub, double *__restrict__ Usup) {
+    double *__restrict__ Lsub, double *__restrict__ Usup, int write_zeros) {
     extern __shared__ double smem[];
     double *yl = smem;                       // ni
     double *dg0 = smem + VK_NI_MAX;          // diffusion terms of the diagonal, per species
@@ -426,7 +426,9 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
         }
     }
     __syncthreads();
-    // Every dense entry is written once.  Entries with at most JAC_LIGHT
+    // Every dense entry is written once (the structural zeros only with
+    // write_zeros: the integrator clears its blocks once and skips them).
+    // Entries with at most JAC_LIGHT
     // terms (and the structural zeros) in one coalesced pass over the dense
     // index; the few long term lists (jac_ent[0, nheavy): the list is sorted
     // by term count) by their own threads alongside.  With few layers the
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(BLK) k_jac_lhs(
     for (int e = e0; e < nn; e += es) {
         // (first term << 4 | term count), -1: structural zero, -2: long term list
         const int code = jac_emap[e];
-        if (code == -2) continue;
+        if (code == -2 || (code == -1 && !write_zeros)) continue;
         double val = 0.0;
         if (code >= 0) {
             const int u0 = code >> 4, nt = code & 15;
@@ -520,7 +522,7 @@ int vk_launch_rhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm
 int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm *atm, const double *y,
                       const double *k, const double *M, int mode, int mol, const double *dtc,
                       const vk_colstate *st, double *ysum, double *D, double *Lsub, double *Usup, hipStream_t s,
-                      bool ysum_ready = false) {
+                      bool ysum_ready = false, bool write_zeros = true) {
     vk_atm a{};
     if (atm) a = *atm;
     if (mode == 1) {
@@ -542,7 +544,7 @@ int vk_launch_jac_lhs(const vk_plan *plan, const vk_columns *cols, const vk_atm 
     hipLaunchKernelGGL(k_jac_lhs, dim3(cols->L, gy), dim3(BLK), jac_smem(plan), s, plan->ni, plan->npairs, plan->nslot,
                        plan->njent, plan->njac, cols->d_off, cols->d_layer_col, plan->reac, plan->prod, plan->slot_pair,
                        plan->slot_sp, plan->jac_ent, plan->jac_emap, plan->njheavy, plan->jac_tp, plan->ms, plan->alpha, a, y, k, M, mode, mol,
-                       dtc, ysum, st, D, Lsub, Usup);
+                       dtc, ysum, st, D, Lsub, Usup, (int)write_zeros);
     VK_CHECK(hipGetLastError());
     return 0;
 }

@@ -15,7 +15,7 @@ int vk_launch_rhs(const vk_plan *, const vk_columns *, const vk_atm *, const dou
                   const double *, const double *, const vk_colstate *, double *, double *, hipStream_t);
 int vk_launch_jac_lhs(const vk_plan *, const vk_columns *, const vk_atm *, const double *, const double *,
                       const double *, int, int, const double *, const vk_colstate *, double *, double *, double *,
-                      double *, hipStream_t, bool ysum_ready = false);
+                      double *, hipStream_t, bool ysum_ready = false, bool write_zeros = true);
 int vk_launch_factor(const vk_plan *, const vk_columns *, const double *, const double *, const double *, double *,
                      int *, const vk_colstate *, const vk_bcr_ws *, hipStream_t, bool info_zeroed = false);
 int vk_launch_solve_refined(const vk_plan *, const vk_columns *, const double *, const double *, const double *,
@@ -746,6 +746,10 @@ extern "C" int vk_integrator_create(const vk_plan *plan, const vk_columns *cols,
     auto &v = it->allocs;
     int r = 0;
     r = r ? r : dalloc(v, &it->D, L * ni * ni);
+    // the blocks' structural zeros, written once: the lhs assembly of an
+    // attempt writes only the entries that can be nonzero (nothing else
+    // writes D)
+    if (!r && hipMemset(it->D, 0, sizeof(double) * L * ni * ni) != hipSuccess) r = VK_ENOMEM;
     r = r ? r : dalloc(v, &it->Dinv, L * ni * ni);
     r = r ? r : dalloc(v, &it->Lsub, L * ni);
     r = r ? r : dalloc(v, &it->Usup, L * ni);
@@ -862,7 +866,7 @@ static int ros2_attempt(vk_integrator *it, const vk_atm *atm, const vk_params *p
     if (r) return r;
     // the right-hand side above has left the layer sums of y in it->ysum
     r = vk_launch_jac_lhs(p, cl, atm, y, k, atm->M, 1, prm->use_moldiff, dt, st, it->ysum, it->D, it->Lsub, it->Usup,
-                          s, true);
+                          s, true, false);
     if (r) return r;
     if (timed) VK_CHECK(hipEventRecord(it->ev0, s));
     r = vk_launch_factor(p, cl, it->D, it->Lsub, it->Usup, it->Dinv, it->info, st, &it->bcr, s, true);
