@@ -414,6 +414,69 @@ __global__ void __launch_bounds__(MV_NT) k_bcr_bwd_s(int ni, int n, const int4 *
     }
 }
 
+// The last level of one column when it holds two layers, the top survivor
+// s and the eliminated layer e to its right: r_s -= Q_s r_e, x_s = A_s^-1
+// r_s, x_e = A_e^-1 r_e - XB_e x_s, in one workgroup of MV_NT-thread row
+// groups (ni <= 96) instead of three launches.  Each product is formed as
+// in k_bcr_fwd_s / k_bcr_bwd_s (same partition of the columns, same order
+// of the sums; their product with an absent neighbour is exactly +0.0 and
+// stands here as that constant), so the results are theirs.
+constexpr int TAIL_G = 3, TAIL_NIMAX = TAIL_G * MV_RG;
+__global__ void __launch_bounds__(MV_NT *TAIL_G) k_bcr_tail_s(int ni, const int4 *__restrict__ sv, int slot0,
+                                                              const int4 *__restrict__ el, const double *__restrict__ Q,
+                                                              const double *__restrict__ Dinv,
+                                                              const double *__restrict__ XB, double *r, double *x,
+                                                              const vk_colstate *__restrict__ st, double *acc) {
+    const int4 es = sv[0], ee = el[0];
+    if (st && st[es.w].done) return;
+    const size_t nn = (size_t)ni * ni;
+    const int g = threadIdx.x / MV_NT, tt = threadIdx.x % MV_NT;
+    const int lr = tt % MV_RG, a = MV_RG * g + lr, h = tt / MV_RG, ch = (ni + MV_SP - 1) / MV_SP;
+    const int ks = es.x, ke = ee.x;
+    // every operator up front: Q_s, A_s^-1, A_e^-1, XB_e
+    double mq[MV_CH], mxs[MV_CH], mxe[MV_CH], mb[MV_CH];
+    mv_load(mq, Q + (size_t)slot0 * nn, ni, a, h, ch);
+    mv_load(mxs, Dinv + (size_t)ks * nn, ni, a, h, ch);
+    mv_load(mxe, Dinv + (size_t)ke * nn, ni, a, h, ch);
+    mv_load(mb, XB + (size_t)ke * nn, ni, a, h, ch);
+    __shared__ double re[TAIL_NIMAX], rs[TAIL_NIMAX], xs[TAIL_NIMAX], pp[TAIL_G][MV_SP][MV_RG];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) re[i] = r[(size_t)ke * ni + i];
+    __syncthreads();
+    auto total = [&]() {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < MV_SP; k++) t += pp[g][k][lr];
+        return t;
+    };
+    // forward: r_s -= P_s r_l + Q_s r_r, no left neighbour
+    pp[g][h][lr] = 0.0 + mv_dot(mq, re, ni, h, ch);
+    __syncthreads();
+    if (h == 0 && a < ni) {
+        const double v = r[(size_t)ks * ni + a] - total();
+        r[(size_t)ks * ni + a] = v;
+        rs[a] = v;
+    }
+    __syncthreads();
+    // top: x_s = A_s^-1 r_s, no neighbours
+    pp[g][h][lr] = (mv_dot(mxs, rs, ni, h, ch) - 0.0) - 0.0;
+    __syncthreads();
+    if (h == 0 && a < ni) {
+        const double t = total();
+        x[(size_t)ks * ni + a] = t;
+        if (acc) acc[(size_t)ks * ni + a] = acc[(size_t)ks * ni + a] + t;
+        xs[a] = t;
+    }
+    __syncthreads();
+    // back-substitution of e: x_e = A_e^-1 r_e - XB_e x_s, no right neighbour
+    pp[g][h][lr] = (mv_dot(mxe, re, ni, h, ch) - mv_dot(mb, xs, ni, h, ch)) - 0.0;
+    __syncthreads();
+    if (h == 0 && a < ni) {
+        const double t = total();
+        x[(size_t)ke * ni + a] = t;
+        if (acc) acc[(size_t)ke * ni + a] = acc[(size_t)ke * ni + a] + t;
+    }
+}
+
 // the two product launches of a level >= 1
 template <int NB, int KC>
 void launch_bcr_products(int ni, int ns, int ne, int ny, const int4 *sv, const int *sflag, int slot0,
@@ -507,9 +570,17 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
     auto bwd = small ? k_bcr_bwd_s : k_bcr_bwd;
     const int bt = small ? MV_NT : BCR_ST, gy = small ? (ni + MV_RG - 1) / MV_RG : 1;
     if (!small) hipLaunchKernelGGL(k_copy_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, tmp);
+    // the last level and the top in one launch (one column, ni <= 96)
+    const bool tail = small && b.tail && ni <= TAIL_NIMAX;
     for (int lev = 0; lev < b.nlev; lev++) {
         const int s0 = b.surv_off[lev], ns = b.surv_off[lev + 1] - s0;
         if (ns <= 0) continue;
+        if (tail && lev == b.nlev - 1) {
+            const int e0 = b.elim_off[lev];
+            hipLaunchKernelGGL(k_bcr_tail_s, dim3(1), dim3(MV_NT * ((ni + MV_RG - 1) / MV_RG)), 0, s, ni,
+                               b.d_surv + s0, s0, b.d_elim + e0, ws->Q, Dinv, ws->XB, tmp, out, st, acc);
+            break;
+        }
         if (small)
             hipLaunchKernelGGL(k_bcr_fwd_s, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q,
                                lev == 0 ? in : tmp, tmp, lev == 0 ? b.d_sflag + s0 : (const int *)nullptr, st);
@@ -517,9 +588,10 @@ int vk_launch_bcr_solve(int ni, const vk_columns *cols, const double *Dinv, cons
             hipLaunchKernelGGL(k_bcr_fwd, dim3(ns, gy), dim3(bt), 0, s, ni, ns, b.d_surv + s0, s0, ws->P, ws->Q, tmp,
                                st);
     }
-    hipLaunchKernelGGL(bwd, dim3(b.ntop, gy), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, Dinv,
-                       ws->XB, ws->XC, tmp, out, st, acc);
-    for (int lev = b.nlev - 1; lev >= 0; lev--) {
+    if (!tail)
+        hipLaunchKernelGGL(bwd, dim3(b.ntop, gy), dim3(bt), 0, s, ni, b.ntop, (const int4 *)nullptr, b.d_top, Dinv,
+                           ws->XB, ws->XC, tmp, out, st, acc);
+    for (int lev = b.nlev - 1 - (tail ? 1 : 0); lev >= 0; lev--) {
         const int e0 = b.elim_off[lev], ne = b.elim_off[lev + 1] - e0;
         if (ne > 0)
             hipLaunchKernelGGL(bwd, dim3(ne, gy), dim3(bt), 0, s, ni, ne, b.d_elim + e0, (const int2 *)nullptr, Dinv,

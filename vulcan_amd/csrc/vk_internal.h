@@ -55,6 +55,10 @@ struct vk_bcr_sched {
     int4 *d_surv = nullptr;   // (layer, left or -1, right or -1, column)
     int *d_sflag = nullptr;   // per surviving entry: bit 0 new left coupling, bit 1 new right coupling
     int2 *d_top = nullptr;    // (layer, column)
+    // one column whose last level (not level 0) eliminates the layer right
+    // of the top survivor (two active layers): that level's sweeps and the
+    // top solve run as one launch
+    bool tail = false;
 };
 
 // workspace of the block cyclic reduction: A, B, C per layer (n x n each),

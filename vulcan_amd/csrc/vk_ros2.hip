@@ -610,6 +610,11 @@ static int bcr_schedule(vk_columns *c) {
         b.nlev++;
     }
     for (int i = 0; i < c->ncols; i++) top.push_back(make_int2(act[i][0], i));
+    if (c->ncols == 1 && b.nlev >= 2) {   // level 0 reads the input and places the rest: not fused
+        const int e0 = b.elim_off[b.nlev - 1], s0 = b.surv_off[b.nlev - 1];
+        b.tail = (int)el.size() - e0 == 1 && (int)sv.size() - s0 == 1 && sv[s0].x == top[0].x && sv[s0].y < 0 &&
+                 sv[s0].z == el[e0].x && el[e0].y == top[0].x && el[e0].z < 0;
+    }
     b.nsurv = (int)sv.size();
     b.ntop = (int)top.size();
     auto put = [](auto **dst, const auto &vec) -> int {
