@@ -290,7 +290,6 @@ __device__ inline void dd_sub_prod(double &hi, double &lo, double a, double b) {
 // before the first product), and the partials, b and the two coupling
 // terms are combined in double-double.
 constexpr int RES_ROWS = 32, RES_PARTS = NT / RES_ROWS;
-constexpr int RES_CH = (VK_NI_MAX + RES_PARTS - 1) / RES_PARTS;   // longest run
 
 __device__ inline void dd_add(double &hi, double &lo, double a_hi, double a_lo) {
     const double s = hi + a_hi;
@@ -300,6 +299,8 @@ __device__ inline void dd_add(double &hi, double &lo, double a_hi, double a_lo) 
     lo = lo + (e + a_lo);
 }
 
+// CH: the longest run, 16 NB / RES_PARTS for blocks of up to 16 NB rows
+template <int CH>
 __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__ off,
                                                   const int *__restrict__ layer_col, const double *__restrict__ D,
                                                   const double *__restrict__ Lsub, const double *__restrict__ Usup,
@@ -322,16 +323,16 @@ __global__ void __launch_bounds__(NT) k_residual(int ni, const int *__restrict__
     const int m0 = pp * chunk, m1 = min(ni, m0 + chunk);
     // this thread's run of its row (indices clamped into the row; entries
     // past the run are loaded but not used)
-    double dr[RES_CH];
+    double dr[CH];
     const double *row = Dl + (size_t)min(rr, nrows - 1) * ni;
 #pragma unroll
-    for (int k = 0; k < RES_CH; k++) dr[k] = row[min(m0 + k, ni - 1)];
+    for (int k = 0; k < CH; k++) dr[k] = row[min(m0 + k, ni - 1)];
     for (int m = tid; m < ni; m += NT) xs[m] = x[(size_t)l * ni + m];
     __syncthreads();
     double hi = 0.0, lo = 0.0;
     if (rr < nrows) {
 #pragma unroll
-        for (int k = 0; k < RES_CH; k++)
+        for (int k = 0; k < CH; k++)
             if (m0 + k < m1) dd_sub_prod(hi, lo, dr[k], xs[m0 + k]);
     }
     ph[tid] = hi;
@@ -433,8 +434,18 @@ int vk_launch_residual(const vk_plan *plan, const vk_columns *cols, const double
                        hipStream_t s) {
     const int ni = plan->ni;
     const size_t smem = sizeof(double) * ((size_t)ni + 2 * NT);
-    hipLaunchKernelGGL(k_residual, dim3(cols->L, (ni + RES_ROWS - 1) / RES_ROWS), dim3(NT), smem, s, ni, cols->d_off,
-                       cols->d_layer_col, D, Lsub, Usup, x, b, r, st);
+    const dim3 grid(cols->L, (ni + RES_ROWS - 1) / RES_ROWS);
+    // runs of ceil(ni / RES_PARTS) <= ceil(16 nb / RES_PARTS) entries
+    switch ((ni + 15) / 16) {
+#define VK_RES(NB)                                                                                                   \
+    case NB:                                                                                                         \
+        hipLaunchKernelGGL(k_residual<(16 * NB + RES_PARTS - 1) / RES_PARTS>, grid, dim3(NT), smem, s, ni, cols->d_off, \
+                           cols->d_layer_col, D, Lsub, Usup, x, b, r, st);                                           \
+        break;
+        VK_RES(1) VK_RES(2) VK_RES(3) VK_RES(4) VK_RES(5) VK_RES(6) VK_RES(7) VK_RES(8) VK_RES(9) VK_RES(10)
+#undef VK_RES
+    default: return VK_ETOOBIG;
+    }
     VK_CHECK(hipGetLastError());
     return 0;
 }
