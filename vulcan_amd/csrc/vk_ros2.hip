@@ -114,12 +114,13 @@ __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *
     for (int o = 32; o > 0; o >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, o));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
     __syncthreads();
+    // the serial sums on threads of different waves, each in its own order
     if (threadIdx.x == 0) {
         double r = red[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmax(r, red[q]);
         part[(size_t)l * NP + VKP_DELTA] = r;
-        ysum = pw_sum(sol, ni);
     }
+    if (threadIdx.x == 64) ysum = pw_sum(sol, ni);
     __syncthreads();
     const double ys = ysum;
     if (!clip) {
@@ -145,20 +146,25 @@ __global__ void __launch_bounds__(256) k_finalize(int ni, int natom, const int *
     ng[threadIdx.x] = nega;
     negf[threadIdx.x] = anyneg;
     __syncthreads();
+    // threads from ni on hold zero partials (adding +0.0 to a sum that
+    // starts at +0.0 changes nothing); each sum in thread order, the sums on
+    // threads of different waves
+    const int nt = min((int)blockDim.x, ni);
     if (threadIdx.x == 0) {
-        double s1 = 0.0, s2 = 0.0;
-        int nf = 0;
-        // threads from ni on hold zero partials (adding +0.0 to a sum that
-        // starts at +0.0 changes nothing)
-        const int nt = min((int)blockDim.x, ni);
-        for (int t = 0; t < nt; t++) { s1 += sm[t]; s2 += ng[t]; nf |= negf[t]; }
-        part[(size_t)l * NP + VKP_NEG] = nf;
+        double s1 = 0.0;
+        for (int t = 0; t < nt; t++) s1 += sm[t];
         part[(size_t)l * NP + VKP_SMALL] = s1;
+    } else if (threadIdx.x == 64) {
+        double s2 = 0.0;
+        int nf = 0;
+        for (int t = 0; t < nt; t++) { s2 += ng[t]; nf |= negf[t]; }
+        part[(size_t)l * NP + VKP_NEG] = nf;
         part[(size_t)l * NP + VKP_NEGY] = s2;
+    } else if (threadIdx.x == 128) {
         ysum = pw_sum(sol, ni);
     }
-    if (threadIdx.x < natom) {
-        const int a = threadIdx.x;
+    if (threadIdx.x >= 192 && threadIdx.x - 192 < natom) {
+        const int a = threadIdx.x - 192;
         double acc = 0.0;
         for (int i = 0; i < ni; i++) acc += atom_mat[(size_t)i * natom + a] * sol[i];
         part[(size_t)l * NP + VKP_ATOM + a] = acc;
